@@ -1,0 +1,92 @@
+"""GPU parity against the CPU oracle, through the C ABI (libtess.so)."""
+import numpy as np
+import pytest
+
+import tess
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+UNIT_LO, UNIT_HI = np.zeros(3), np.ones(3)
+
+
+def _compare(got, ref, vol_tol=1e-12):
+    assert np.array_equal(got["face_count"], ref["face_count"])
+    assert np.array_equal(got["face_offset"], ref["face_offset"])
+    assert np.array_equal(got["neighbor"], ref["neighbor"])
+    rv = np.abs(got["volume"] - ref["volume"]) / np.abs(ref["volume"])
+    assert rv.max() <= vol_tol, rv.max()
+    ra = np.abs(got["area"] - ref["area"]) / np.maximum(np.abs(ref["area"]), 1e-300)
+    assert ra.max() <= vol_tol, ra.max()
+
+
+def test_arith_correctly_rounded(ctx):
+    rng = np.random.default_rng(5)
+    a = np.concatenate([rng.random(20000), rng.random(20000) * 1e-8, np.exp(rng.normal(0, 30, 20000))])
+    b = np.concatenate([rng.random(20000) + 0.5, np.exp(rng.normal(0, 10, 40000))])
+    s, r, q = ctx.diag_arith(a, b)
+    assert np.array_equal(s, np.sqrt(a))
+    assert np.array_equal(r, 1.0 / a)
+    assert np.array_equal(q, a / b)
+
+
+@pytest.mark.parametrize("n,seed,density", [(10_000, 0, 1.25), (200_000, 1, 5.0)])
+def test_binning_matches_oracle(ctx, n, seed, density):
+    pts = tess.uniform(n, seed)
+    d = tess.Diagram(ctx, pts, UNIT_LO, UNIT_HI, density=density)
+    d.compute()
+    b, s, cpd = d.grid()
+    g, bins, srt, dl = orc.grid(pts, density)
+    assert cpd == g.cpd
+    assert np.array_equal(b, [g.xmin, g.xmax, g.ymin, g.ymax, g.zmin, g.zmax])
+    assert np.array_equal(s, [g.sx, g.sy, g.sz])
+    cell, sorted_, delim = d.copy_grid()
+    assert np.array_equal(cell, bins)
+    assert np.array_equal(sorted_, srt)
+    assert np.array_equal(delim, dl)
+
+
+def test_single_point_closed_form(ctx):
+    d = tess.Diagram(ctx, np.array([[0.3, 0.6, 0.2]]), UNIT_LO, UNIT_HI)
+    d.compute()
+    c = d.cells()
+    assert c["volume"][0] == 1.0
+    assert list(c["neighbor"]) == [-6, -5, -4, -3, -2, -1]
+    assert np.all(c["area"] == 1.0)
+
+
+def test_two_points_closed_form(ctx):
+    d = tess.Diagram(ctx, np.array([[0.25, 0.5, 0.5], [0.75, 0.5, 0.5]]), UNIT_LO, UNIT_HI)
+    d.compute()
+    c = d.cells()
+    assert np.array_equal(c["volume"], [0.5, 0.5])
+    assert list(c["face_count"]) == [6, 6]
+    assert list(c["neighbor"][:6]) == [-6, -5, -4, -3, -1, 1]
+    assert list(c["neighbor"][6:]) == [-6, -5, -4, -3, -2, 0]
+
+
+@pytest.mark.parametrize("m", [4, 6])
+def test_cubic_lattice(ctx, m):
+    ax = (np.arange(m) + 0.5) / m
+    pts = np.array([[a, b, c] for a in ax for b in ax for c in ax])
+    d = tess.Diagram(ctx, pts, UNIT_LO, UNIT_HI)
+    d.compute()
+    got = d.cells()
+    ref = orc.compute(pts, UNIT_LO, UNIT_HI)
+    _compare(got, ref)
+    assert np.allclose(got["volume"], 1.0 / m**3, rtol=1e-12, atol=0)
+    assert np.all(got["face_count"] == 6)
+
+
+@pytest.mark.parametrize("n,seed,density", [(100, 3, 1.25), (1000, 4, 1.25), (10_000, 0, 1.25), (20_000, 7, 5.0)])
+def test_uniform_cells_match_oracle(ctx, n, seed, density):
+    pts = tess.uniform(n, seed)
+    d = tess.Diagram(ctx, pts, UNIT_LO, UNIT_HI, density=density, collect_stats=True)
+    d.compute()
+    got = d.cells()
+    ref = orc.compute(pts, UNIT_LO, UNIT_HI, density=density)
+    _compare(got, ref)
+    assert abs(got["volume"].sum() - 1.0) <= 1e-10
+    st, rs = d.stats(), ref["stats"]
+    for k in ("streamed", "classified", "cuts", "entries", "flops"):
+        assert np.array_equal(st[k].astype(np.uint64), rs[k].astype(np.uint64)), k

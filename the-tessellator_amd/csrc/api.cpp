@@ -1,0 +1,861 @@
+// api.cpp -- host runtime behind include/tess.h.
+//
+// Owns device memory and the HIP stream, builds the canonical shell table
+// (celery.rs:418-679) on the host once per grid shape, and sequences the
+// device passes: bounds -> counting sort -> gather -> cell kernel (with a
+// larger-capacity rerun for cells that outgrow the default) -> face offsets
+// -> CSR compaction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "launch.h"
+#include "tess_internal.h"
+
+using namespace tess;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    reset();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * count);
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+};
+
+// ---------------------------------------------------------------- shell table
+// Generation order of celery.rs:437-673: the centre, then offsets with three,
+// two (xy, xz, yz) and one (x, y, z) non-zero components, each block looping
+// over |o|-1 in lexicographic order with the sign patterns listed there.
+uint64_t generation_index(int64_t cpd, int ox, int oy, int oz) {
+  const uint64_t M = (uint64_t)(cpd - 1);
+  const uint64_t a = ox ? (uint64_t)std::abs(ox) - 1 : 0;
+  const uint64_t b = oy ? (uint64_t)std::abs(oy) - 1 : 0;
+  const uint64_t c = oz ? (uint64_t)std::abs(oz) - 1 : 0;
+  const uint64_t two = 1 + 8 * M * M * M, one = two + 12 * M * M;
+  if (!ox && !oy && !oz) return 0;
+  if (ox && oy && oz) {
+    static const int order[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+    const int bits = ((ox < 0) << 2) | ((oy < 0) << 1) | (oz < 0);
+    return 1 + 8 * ((a * M + b) * M + c) + order[bits];
+  }
+  if (ox && oy) return two + 4 * (a * M + b) + (((ox < 0) << 1) | (oy < 0));
+  if (ox && oz) return two + 4 * M * M + 4 * (a * M + c) + (((ox < 0) << 1) | (oz < 0));
+  if (oy && oz) return two + 8 * M * M + 4 * (b * M + c) + (((oy < 0) << 1) | (oz < 0));
+  if (ox) return one + 2 * a + (ox < 0);
+  if (oy) return one + 2 * M + 2 * b + (oy < 0);
+  return one + 4 * M + 2 * c + (oz < 0);
+}
+
+// celery.rs:419-427: squared gap distance, (sq(gx sx) + sq(gy sy)) + sq(gz sz).
+double shell_key(int ox, int oy, int oz, double sx, double sy, double sz) {
+  if (!ox && !oy && !oz) return -1.0;
+  auto sq = [](double t) { return t * t; };
+  const double gx = ox ? std::abs(ox) - 1 : 0, gy = oy ? std::abs(oy) - 1 : 0,
+               gz = oz ? std::abs(oz) - 1 : 0;
+  return sq(gx * sx) + sq(gy * sy) + sq(gz * sz);
+}
+
+struct Table {
+  DevBuf<double> key;
+  DevBuf<int32_t> off;
+  int len = 0;
+  double kcut = 0;
+  int L = 0;
+};
+
+// Entries with |o|_inf <= L sorted by (key, generation index).  For L < cpd-1
+// only keys below kcut = min_a (L s_a)^2 are kept: every offset outside the
+// cube has a key >= kcut, so the result is a prefix of the full order.
+void build_table_host(int64_t cpd, double sx, double sy, double sz, int L, std::vector<double>& key,
+                      std::vector<int32_t>& off, double* kcut) {
+  if (L > cpd - 1) L = (int)(cpd - 1);
+  const bool full = (L == cpd - 1);
+  double kc = std::numeric_limits<double>::infinity();
+  if (!full) {
+    auto sq = [](double t) { return t * t; };
+    kc = std::min(sq(L * sx), std::min(sq(L * sy), sq(L * sz)));
+  }
+  struct E {
+    double k;
+    uint64_t g;
+    int32_t o;
+  };
+  std::vector<E> es;
+  es.reserve((size_t)(2 * L + 1) * (2 * L + 1) * (2 * L + 1));
+  for (int i = -L; i <= L; ++i)
+    for (int j = -L; j <= L; ++j)
+      for (int k = -L; k <= L; ++k) {
+        const double kk = shell_key(i, j, k, sx, sy, sz);
+        if (!full && !(kk < kc)) continue;
+        es.push_back({kk, generation_index(cpd, i, j, k), pack_off(i, j, k)});
+      }
+  std::sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.k != b.k ? a.k < b.k : a.g < b.g; });
+  key.resize(es.size());
+  off.resize(es.size());
+  for (size_t t = 0; t < es.size(); ++t) {
+    key[t] = es[t].k;
+    off[t] = es[t].o;
+  }
+  *kcut = kc;
+}
+
+constexpr int kTableL = 12;
+
+uint64_t bits(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, 8);
+  return u;
+}
+
+int64_t icbrt_floor(double x) {
+  if (!(x >= 1.0)) return 0;
+  int64_t k = (int64_t)std::cbrt(x);
+  while (k > 0 && (double)k * (double)k * (double)k > x) --k;
+  while ((double)(k + 1) * (double)(k + 1) * (double)(k + 1) <= x) ++k;
+  return k;
+}
+
+}  // namespace
+
+struct tess_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mu;
+  // shell tables keyed by (cpd, sx, sy, sz, L)
+  std::map<std::tuple<int64_t, uint64_t, uint64_t, uint64_t, int>, std::unique_ptr<Table>> tables;
+  DevBuf<double> diag;
+
+  tess_status fail(tess_status s, const std::string& m) {
+    err = m;
+    return s;
+  }
+  tess_status hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return TESS_OK;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return TESS_E_DEVICE;
+  }
+  Table* table(int64_t cpd, double sx, double sy, double sz, int L, tess_status* st) {
+    auto key = std::make_tuple(cpd, bits(sx), bits(sy), bits(sz), L);
+    auto it = tables.find(key);
+    if (it != tables.end()) return it->second.get();
+    std::vector<double> k;
+    std::vector<int32_t> o;
+    double kc;
+    build_table_host(cpd, sx, sy, sz, L, k, o, &kc);
+    auto t = std::make_unique<Table>();
+    if (t->key.alloc(k.size()) != hipSuccess || t->off.alloc(o.size()) != hipSuccess) {
+      *st = fail(TESS_E_NOMEM, "shell table allocation failed");
+      return nullptr;
+    }
+    if (hipMemcpyAsync(t->key.p, k.data(), 8 * k.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(t->off.p, o.data(), 4 * o.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      *st = fail(TESS_E_DEVICE, "shell table upload failed");
+      return nullptr;
+    }
+    t->len = (int)k.size();
+    t->kcut = kc;
+    t->L = L;
+    Table* r = t.get();
+    tables[key] = std::move(t);
+    return r;
+  }
+};
+
+struct tess_cell {
+  int64_t original = -1;
+  double volume = 0;
+  std::vector<int64_t> nbr;
+  std::vector<double> area;
+  std::vector<std::vector<double>> ring;  // site-relative coordinates per face
+  std::vector<double> verts;
+};
+
+struct tess_diagram {
+  tess_ctx* ctx = nullptr;
+  tess_opts opts;
+  size_t n = 0;
+  int64_t cpd = 0;
+  int nbins = 0;
+  double lo[3], hi[3];
+  bool have_groups = false;
+  // inputs
+  DevBuf<double> x, y, z, xyz_tmp;
+  DevBuf<int64_t> groups;
+  // grid
+  DevBuf<GridDev> grid;
+  DevBuf<double> part;
+  DevBuf<uint32_t> bins, cnt, delim, cursor, sorted, scratch, sums, big, misc, inv;
+  DevBuf<double> sx, sy, sz;
+  GridDev hgrid;
+  bool binned = false;
+  bool computed = false;
+  bool inv_ready = false;
+  // outputs
+  DevBuf<double> volume, sarea, area;
+  DevBuf<uint32_t> fcount, status, redo;
+  DevBuf<int32_t> snbr, nbr;
+  DevBuf<uint64_t> foff, fsums;
+  DevBuf<tess_cell_stats> stats;
+  DevBuf<uint32_t> work1;
+  DevBuf<double> exp_v;
+  DevBuf<int32_t> exp_ring, exp_meta;
+  uint64_t total_faces = 0;
+  tess_timings tim;
+  uint64_t degenerate = 0;
+};
+
+namespace {
+
+tess_status alloc_all(tess_diagram* d) {
+  tess_ctx* c = d->ctx;
+  const size_t n = d->n;
+  const int nparts = (int)std::min<size_t>(1024, (n + 255) / 256);
+  const size_t nb = (size_t)d->nbins;
+  bool ok = d->x.alloc(n) == hipSuccess && d->y.alloc(n) == hipSuccess && d->z.alloc(n) == hipSuccess &&
+            d->grid.alloc(1) == hipSuccess && d->part.alloc(6 * (size_t)nparts) == hipSuccess &&
+            d->bins.alloc(n) == hipSuccess && d->cnt.alloc(nb) == hipSuccess &&
+            d->delim.alloc(nb + 1) == hipSuccess && d->cursor.alloc(nb) == hipSuccess &&
+            d->sorted.alloc(n) == hipSuccess && d->scratch.alloc(n) == hipSuccess &&
+            d->sums.alloc((nb + 1023) / 1024 + 1) == hipSuccess && d->big.alloc(nb) == hipSuccess &&
+            d->misc.alloc(4) == hipSuccess && d->sx.alloc(n) == hipSuccess && d->sy.alloc(n) == hipSuccess &&
+            d->sz.alloc(n) == hipSuccess && d->volume.alloc(n) == hipSuccess &&
+            d->fcount.alloc(n) == hipSuccess && d->status.alloc(n) == hipSuccess &&
+            d->redo.alloc(n + 1) == hipSuccess && d->snbr.alloc(n * kFaceStage) == hipSuccess &&
+            d->sarea.alloc(n * kFaceStage) == hipSuccess && d->foff.alloc(n + 1) == hipSuccess &&
+            d->fsums.alloc((n + 1023) / 1024 + 1) == hipSuccess && d->work1.alloc(1) == hipSuccess;
+  if (ok && d->opts.collect_stats) ok = d->stats.alloc(n) == hipSuccess;
+  if (!ok) return c->fail(TESS_E_NOMEM, "device allocation failed");
+  return TESS_OK;
+}
+
+tess_status bin(tess_diagram* d) {
+  tess_ctx* c = d->ctx;
+  hipStream_t s = c->stream;
+  const int n = (int)d->n;
+  const int nparts = (int)std::min<size_t>(1024, (d->n + 255) / 256);
+  tess_status st;
+  if ((st = c->hip(launch_bounds(d->x.p, d->y.p, d->z.p, n, (int)d->cpd, d->part.p, nparts, d->grid.p, s),
+                   "bounds")) != TESS_OK)
+    return st;
+  SortBufs b{d->bins.p, d->cnt.p, d->delim.p, d->cursor.p, d->sorted.p, d->scratch.p, d->sums.p, d->big.p,
+             d->misc.p};
+  if ((st = c->hip(launch_counting_sort(d->x.p, d->y.p, d->z.p, n, d->grid.p, d->nbins, b, s), "sort")) !=
+      TESS_OK)
+    return st;
+  if ((st = c->hip(launch_gather(d->sorted.p, n, d->x.p, d->y.p, d->z.p, d->sx.p, d->sy.p, d->sz.p, s),
+                   "gather")) != TESS_OK)
+    return st;
+  if ((st = c->hip(hipMemcpyAsync(&d->hgrid, d->grid.p, sizeof(GridDev), hipMemcpyDeviceToHost, s),
+                   "grid copy")) != TESS_OK)
+    return st;
+  // every point must have landed in exactly one grid cell before any pass
+  // dereferences the sorted indices
+  uint32_t total = 0;
+  if ((st = c->hip(hipMemcpyAsync(&total, d->delim.p + d->nbins, 4, hipMemcpyDeviceToHost, s), "grid copy")) !=
+          TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "binning")) != TESS_OK)
+    return st;
+  if (total != (uint32_t)d->n) return c->fail(TESS_E_DEVICE, "counting sort lost points");
+  d->binned = true;
+  d->inv_ready = false;
+  return TESS_OK;
+}
+
+int pick_vpl(const tess_diagram* d) {
+  switch (d->opts.vertex_capacity) {
+    case 64:
+      return 1;
+    case 256:
+      return 4;
+    default:
+      return 2;
+  }
+}
+
+CellArgs base_args(tess_diagram* d, const Table* t) {
+  CellArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = (int)d->n;
+  a.spx = d->sx.p;
+  a.spy = d->sy.p;
+  a.spz = d->sz.p;
+  a.sorted = d->sorted.p;
+  a.delim = d->delim.p;
+  a.grid = d->grid.p;
+  a.tkey = t->key.p;
+  a.toff = t->off.p;
+  a.tlen = t->len;
+  a.kcut = t->kcut;
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = d->lo[k];
+    a.hi[k] = d->hi[k];
+  }
+  a.groups = d->have_groups ? d->groups.p : nullptr;
+  a.target_group = -1;
+  a.radius_limit = std::numeric_limits<double>::quiet_NaN();
+  a.volume = d->volume.p;
+  a.fcount = d->fcount.p;
+  a.snbr = d->snbr.p;
+  a.sarea = d->sarea.p;
+  a.status = d->status.p;
+  a.stats = d->opts.collect_stats ? d->stats.p : nullptr;
+  return a;
+}
+
+tess_status create_common(tess_ctx* ctx, size_t n, const double lo[3], const double hi[3],
+                          const tess_opts* opts, tess_diagram** out) {
+  if (!ctx || !out || !lo || !hi) return TESS_E_INVALID;
+  *out = nullptr;
+  if (n == 0) return ctx->fail(TESS_E_INVALID, "empty point set");
+  if (n > (size_t)INT32_MAX / kFaceStage) return ctx->fail(TESS_E_INVALID, "too many points");
+  for (int k = 0; k < 3; ++k)
+    if (!(hi[k] > lo[k])) return ctx->fail(TESS_E_INVALID, "container box is empty");
+  tess_diagram* d = new (std::nothrow) tess_diagram;
+  if (!d) return ctx->fail(TESS_E_NOMEM, "host allocation failed");
+  d->ctx = ctx;
+  if (opts)
+    d->opts = *opts;
+  else
+    tess_opts_default(&d->opts);
+  if (!(d->opts.cell_density > 0)) d->opts.cell_density = 1.25;
+  d->n = n;
+  d->cpd = icbrt_floor((double)n / d->opts.cell_density) + 1;  // celery.rs:161-162
+  if (d->cpd > 1000) {
+    delete d;
+    return ctx->fail(TESS_E_INVALID, "grid too fine (cell_density too small)");
+  }
+  d->nbins = (int)(d->cpd * d->cpd * d->cpd);
+  for (int k = 0; k < 3; ++k) {
+    d->lo[k] = lo[k];
+    d->hi[k] = hi[k];
+  }
+  std::memset(&d->tim, 0, sizeof(d->tim));
+  tess_status st = alloc_all(d);
+  if (st != TESS_OK) {
+    delete d;
+    return st;
+  }
+  *out = d;
+  return TESS_OK;
+}
+
+tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, int nwork) {
+  tess_ctx* c = d->ctx;
+  CellArgs a = base_args(d, t);
+  a.work = work;
+  a.nwork = nwork;
+  a.xcd_remap = work ? 0 : 1;
+  a.redo = d->redo.p + 1;
+  a.nredo = d->redo.p;
+  tess_status st;
+  if ((st = c->hip(hipMemsetAsync(d->redo.p, 0, sizeof(uint32_t), c->stream), "memset")) != TESS_OK) return st;
+  return c->hip(launch_cells(vpl, a, c->stream), "cell kernel");
+}
+
+}  // namespace
+
+extern "C" {
+
+void tess_opts_default(tess_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->cell_density = 1.25;
+  o->mode = TESS_MODE_PLAIN;
+  o->vertex_capacity = 0;
+  o->collect_stats = 0;
+}
+
+tess_status tess_ctx_create(int dev, tess_ctx** out) {
+  if (!out) return TESS_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) return TESS_E_DEVICE;
+  if (hipSetDevice(dev) != hipSuccess) return TESS_E_DEVICE;
+  tess_ctx* c = new (std::nothrow) tess_ctx;
+  if (!c) return TESS_E_NOMEM;
+  c->device = dev;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return TESS_E_DEVICE;
+  }
+  *out = c;
+  return TESS_OK;
+}
+
+void tess_ctx_destroy(tess_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->tables.clear();
+  c->diag.reset();
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* tess_last_error(const tess_ctx* c) { return c ? c->err.c_str() : "null context"; }
+void* tess_ctx_stream(tess_ctx* c) { return c ? (void*)c->stream : nullptr; }
+tess_status tess_ctx_synchronize(tess_ctx* c) {
+  if (!c) return TESS_E_INVALID;
+  return c->hip(hipStreamSynchronize(c->stream), "synchronize");
+}
+
+tess_status tess_diagram_create(tess_ctx* ctx, const double* xyz, size_t n, const double lo[3],
+                                const double hi[3], const int64_t* groups, const tess_opts* opts,
+                                tess_diagram** out) {
+  if (!ctx) return TESS_E_INVALID;
+  if (!xyz) return ctx->fail(TESS_E_INVALID, "null points");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  tess_status st = create_common(ctx, n, lo, hi, opts, out);
+  if (st != TESS_OK) return st;
+  tess_diagram* d = *out;
+  if (d->xyz_tmp.alloc(3 * n) != hipSuccess) {
+    delete d;
+    *out = nullptr;
+    return ctx->fail(TESS_E_NOMEM, "device allocation failed");
+  }
+  hipStream_t s = ctx->stream;
+  st = ctx->hip(hipMemcpyAsync(d->xyz_tmp.p, xyz, 24 * n, hipMemcpyHostToDevice, s), "upload");
+  if (st == TESS_OK) st = ctx->hip(launch_soa(d->xyz_tmp.p, (int)n, d->x.p, d->y.p, d->z.p, s), "soa");
+  if (st == TESS_OK && groups) {
+    d->have_groups = true;
+    if (d->groups.alloc(n) != hipSuccess)
+      st = ctx->fail(TESS_E_NOMEM, "device allocation failed");
+    else
+      st = ctx->hip(hipMemcpyAsync(d->groups.p, groups, 8 * n, hipMemcpyHostToDevice, s), "upload");
+  }
+  if (st == TESS_OK) st = ctx->hip(hipStreamSynchronize(s), "upload");
+  d->xyz_tmp.reset();
+  if (st != TESS_OK) {
+    delete d;
+    *out = nullptr;
+  }
+  return st;
+}
+
+tess_status tess_diagram_create_device(tess_ctx* ctx, const double* dxyz, size_t n, const double lo[3],
+                                       const double hi[3], const int64_t* dgroups, const tess_opts* opts,
+                                       tess_diagram** out) {
+  if (!ctx) return TESS_E_INVALID;
+  if (!dxyz) return ctx->fail(TESS_E_INVALID, "null points");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  tess_status st = create_common(ctx, n, lo, hi, opts, out);
+  if (st != TESS_OK) return st;
+  tess_diagram* d = *out;
+  hipStream_t s = ctx->stream;
+  st = ctx->hip(launch_soa(dxyz, (int)n, d->x.p, d->y.p, d->z.p, s), "soa");
+  if (st == TESS_OK && dgroups) {
+    d->have_groups = true;
+    if (d->groups.alloc(n) != hipSuccess)
+      st = ctx->fail(TESS_E_NOMEM, "device allocation failed");
+    else
+      st = ctx->hip(hipMemcpyAsync(d->groups.p, dgroups, 8 * n, hipMemcpyDeviceToDevice, s), "copy");
+  }
+  if (st == TESS_OK) st = ctx->hip(hipStreamSynchronize(s), "soa");
+  if (st != TESS_OK) {
+    delete d;
+    *out = nullptr;
+  }
+  return st;
+}
+
+void tess_diagram_destroy(tess_diagram* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->ctx->device);
+  (void)hipStreamSynchronize(d->ctx->stream);
+  delete d;
+}
+
+size_t tess_diagram_size(const tess_diagram* d) { return d ? d->n : 0; }
+
+tess_status tess_diagram_compute(tess_diagram* d) {
+  if (!d) return TESS_E_INVALID;
+  tess_ctx* c = d->ctx;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  tess_status st;
+  hipEvent_t ev[4];
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return c->fail(TESS_E_DEVICE, "event create failed");
+  auto cleanup = [&]() {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  };
+  (void)hipEventRecord(ev[0], s);
+  if ((st = bin(d)) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  (void)hipEventRecord(ev[1], s);
+  if ((st = c->hip(hipStreamSynchronize(s), "binning")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const GridDev& G = d->hgrid;
+  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
+  auto t1 = std::chrono::steady_clock::now();
+  if (!t) {
+    cleanup();
+    return st;
+  }
+  const int n = (int)d->n;
+  const int vpl = pick_vpl(d);
+  if ((st = run_cells(d, t, vpl, nullptr, n)) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  // cells that outgrew the capacity: rerun with the largest variant
+  uint32_t nredo = 0;
+  if ((st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  d->tim.capacity_reruns = nredo;
+  if (nredo && vpl < 4) {
+    DevBuf<uint32_t> list;
+    if (list.alloc(nredo) != hipSuccess) {
+      cleanup();
+      return c->fail(TESS_E_NOMEM, "device allocation failed");
+    }
+    (void)hipMemcpyAsync(list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s);
+    if ((st = run_cells(d, t, 4, list.p, (int)nredo)) != TESS_OK ||
+        (st = c->hip(hipStreamSynchronize(s), "cell kernel rerun")) != TESS_OK) {
+      cleanup();
+      return st;
+    }
+  }
+  (void)hipEventRecord(ev[2], s);
+  if ((st = c->hip(launch_face_offsets(d->fcount.p, n, d->foff.p, d->fsums.p, s), "face offsets")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  uint64_t total = 0;
+  if ((st = c->hip(hipMemcpyAsync(&total, d->foff.p + n, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  if (d->nbr.alloc(total) != hipSuccess || d->area.alloc(total) != hipSuccess) {
+    cleanup();
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  }
+  if ((st = c->hip(launch_compact(d->fcount.p, d->foff.p, n, d->snbr.p, d->sarea.p, d->nbr.p, d->area.p, s),
+                   "compact")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  (void)hipEventRecord(ev[3], s);
+  if ((st = c->hip(hipStreamSynchronize(s), "compute")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
+  float ms;
+  (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+  d->tim.bin_ms = ms;
+  (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
+  d->tim.cells_ms = ms;
+  (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
+  d->tim.compact_ms = ms;
+  (void)hipEventElapsedTime(&ms, ev[0], ev[3]);
+  d->tim.total_ms = ms;
+  d->tim.table_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  cleanup();
+  d->total_faces = total;
+  d->computed = true;
+  // status scan: capacity/degenerate cells are reported
+  std::vector<uint32_t> stv(d->n);
+  if ((st = c->hip(hipMemcpy(stv.data(), d->status.p, 4 * d->n, hipMemcpyDeviceToHost), "status")) != TESS_OK)
+    return st;
+  uint64_t bad = 0, cap = 0, tab = 0;
+  for (uint32_t v : stv) {
+    bad += (v & TESS_CELL_DEGENERATE) != 0;
+    cap += (v & TESS_CELL_CAPACITY) != 0;
+    tab += (v & TESS_CELL_TABLE) != 0;
+  }
+  d->degenerate = bad;
+  if (d->opts.collect_stats) {
+    std::vector<tess_cell_stats> sv(d->n);
+    if (hipMemcpy(sv.data(), d->stats.p, sizeof(tess_cell_stats) * d->n, hipMemcpyDeviceToHost) == hipSuccess) {
+      uint64_t f = 0;
+      for (auto& x : sv) f += x.flops;
+      d->tim.flops = f;
+    }
+  }
+  if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
+  if (tab) return c->fail(TESS_E_CAPACITY, std::to_string(tab) + " cells exhausted the shell table");
+  if (bad) return c->fail(TESS_E_DEGENERATE, std::to_string(bad) + " cells failed topology checks");
+  return TESS_OK;
+}
+
+tess_status tess_diagram_total_faces(const tess_diagram* d, uint64_t* out) {
+  if (!d || !out) return TESS_E_INVALID;
+  if (!d->computed) return TESS_E_STATE;
+  *out = d->total_faces;
+  return TESS_OK;
+}
+
+tess_status tess_diagram_copy_cells(const tess_diagram* d, double* volume, uint32_t* fc, uint64_t* fo,
+                                    int32_t* nbr, double* area) {
+  if (!d) return TESS_E_INVALID;
+  if (!d->computed) return d->ctx->fail(TESS_E_STATE, "results read before compute");
+  tess_ctx* c = d->ctx;
+  (void)hipSetDevice(c->device);
+  const size_t n = d->n;
+  hipError_t e = hipSuccess;
+  if (volume && e == hipSuccess) e = hipMemcpy(volume, d->volume.p, 8 * n, hipMemcpyDeviceToHost);
+  if (fc && e == hipSuccess) e = hipMemcpy(fc, d->fcount.p, 4 * n, hipMemcpyDeviceToHost);
+  if (fo && e == hipSuccess) e = hipMemcpy(fo, d->foff.p, 8 * (n + 1), hipMemcpyDeviceToHost);
+  if (nbr && e == hipSuccess && d->total_faces)
+    e = hipMemcpy(nbr, d->nbr.p, 4 * d->total_faces, hipMemcpyDeviceToHost);
+  if (area && e == hipSuccess && d->total_faces)
+    e = hipMemcpy(area, d->area.p, 8 * d->total_faces, hipMemcpyDeviceToHost);
+  return c->hip(e, "download");
+}
+
+tess_status tess_diagram_device_cells(const tess_diagram* d, tess_cells_device* out) {
+  if (!d || !out) return TESS_E_INVALID;
+  if (!d->computed) return TESS_E_STATE;
+  out->volume = d->volume.p;
+  out->face_count = d->fcount.p;
+  out->face_offset = d->foff.p;
+  out->neighbor = d->nbr.p;
+  out->face_area = d->area.p;
+  out->total_faces = d->total_faces;
+  out->n = d->n;
+  return TESS_OK;
+}
+
+tess_status tess_diagram_copy_stats(const tess_diagram* d, tess_cell_stats* stats) {
+  if (!d || !stats) return TESS_E_INVALID;
+  if (!d->computed || !d->opts.collect_stats)
+    return d->ctx->fail(TESS_E_STATE, "stats need collect_stats = 1 and a computed diagram");
+  (void)hipSetDevice(d->ctx->device);
+  return d->ctx->hip(hipMemcpy(stats, d->stats.p, sizeof(tess_cell_stats) * d->n, hipMemcpyDeviceToHost),
+                     "download");
+}
+
+tess_status tess_diagram_timings(const tess_diagram* d, tess_timings* out) {
+  if (!d || !out) return TESS_E_INVALID;
+  *out = d->tim;
+  return TESS_OK;
+}
+
+tess_status tess_diagram_grid(const tess_diagram* d, double b[6], double sz[3], int64_t* cpd) {
+  if (!d) return TESS_E_INVALID;
+  if (!d->binned) return d->ctx->fail(TESS_E_STATE, "grid read before compute");
+  const GridDev& G = d->hgrid;
+  if (b) {
+    b[0] = G.xmin;
+    b[1] = G.xmax;
+    b[2] = G.ymin;
+    b[3] = G.ymax;
+    b[4] = G.zmin;
+    b[5] = G.zmax;
+  }
+  if (sz) {
+    sz[0] = G.sx;
+    sz[1] = G.sy;
+    sz[2] = G.sz;
+  }
+  if (cpd) *cpd = d->cpd;
+  return TESS_OK;
+}
+
+tess_status tess_diagram_copy_grid(const tess_diagram* d, uint32_t* cell_of_point, uint32_t* sorted,
+                                   uint64_t* delim) {
+  if (!d) return TESS_E_INVALID;
+  if (!d->binned) return d->ctx->fail(TESS_E_STATE, "grid read before compute");
+  (void)hipSetDevice(d->ctx->device);
+  hipError_t e = hipSuccess;
+  if (cell_of_point) e = hipMemcpy(cell_of_point, d->bins.p, 4 * d->n, hipMemcpyDeviceToHost);
+  if (sorted && e == hipSuccess) e = hipMemcpy(sorted, d->sorted.p, 4 * d->n, hipMemcpyDeviceToHost);
+  if (delim && e == hipSuccess) {
+    std::vector<uint32_t> tmp((size_t)d->nbins + 1);
+    e = hipMemcpy(tmp.data(), d->delim.p, 4 * tmp.size(), hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < tmp.size(); ++i) delim[i] = tmp[i];
+  }
+  return d->ctx->hip(e, "download");
+}
+
+tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radius, int64_t target_group,
+                               tess_cell** out) {
+  if (!d || !out) return TESS_E_INVALID;
+  *out = nullptr;
+  tess_ctx* c = d->ctx;
+  if (index >= d->n) return c->fail(TESS_E_INVALID, "index out of range");
+  if (target_group >= 0 && !d->have_groups) return c->fail(TESS_E_INVALID, "diagram has no groups");
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  tess_status st;
+  if (!d->binned) {
+    if ((st = bin(d)) != TESS_OK) return st;
+    if ((st = c->hip(hipStreamSynchronize(s), "binning")) != TESS_OK) return st;
+  }
+  if (!d->inv_ready) {
+    if (d->inv.alloc(d->n) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if ((st = c->hip(launch_invert(d->sorted.p, (int)d->n, d->inv.p, s), "invert")) != TESS_OK) return st;
+    d->inv_ready = true;
+  }
+  if (d->exp_v.alloc(3 * 256) != hipSuccess || d->exp_meta.alloc(2 + 256) != hipSuccess ||
+      d->exp_ring.alloc(kFaceStage * 64) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  const GridDev& G = d->hgrid;
+  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
+  if (!t) return st;
+  const uint32_t oi = (uint32_t)index;
+  uint32_t tries[2] = {2, 4};
+  tess_cell* cell = new (std::nothrow) tess_cell;
+  if (!cell) return c->fail(TESS_E_NOMEM, "host allocation failed");
+  uint32_t status = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    CellArgs a = base_args(d, t);
+    a.work = d->work1.p;
+    a.nwork = 1;
+    a.xcd_remap = 0;
+    a.target_group = target_group;
+    a.radius_limit = search_radius;
+    a.exp_v = d->exp_v.p;
+    a.exp_ring = d->exp_ring.p;
+    a.exp_meta = d->exp_meta.p;
+    a.redo = nullptr;
+    a.nredo = nullptr;
+    if ((st = c->hip(hipMemcpyAsync(d->work1.p, d->inv.p + oi, 4, hipMemcpyDeviceToDevice, s), "copy")) !=
+            TESS_OK ||
+        (st = c->hip(launch_cells((int)tries[attempt], a, s), "cell kernel")) != TESS_OK ||
+        (st = c->hip(hipMemcpyAsync(&status, d->status.p + oi, 4, hipMemcpyDeviceToHost, s), "copy")) !=
+            TESS_OK ||
+        (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK) {
+      delete cell;
+      return st;
+    }
+    if (!(status & TESS_CELL_CAPACITY)) break;
+  }
+  if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_TABLE)) {
+    delete cell;
+    return c->fail(status & TESS_CELL_DEGENERATE ? TESS_E_DEGENERATE : TESS_E_CAPACITY,
+                   "cell " + std::to_string(index) + " failed (status " + std::to_string(status) + ")");
+  }
+  uint32_t fc = 0;
+  std::vector<int32_t> nb(kFaceStage), ring(kFaceStage * 64), meta(2 + 256);
+  std::vector<double> ar(kFaceStage), vv(3 * 256);
+  hipError_t e = hipMemcpy(&cell->volume, d->volume.p + oi, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&fc, d->fcount.p + oi, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nb.data(), d->snbr.p + (size_t)oi * kFaceStage, 4 * kFaceStage, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ar.data(), d->sarea.p + (size_t)oi * kFaceStage, 8 * kFaceStage, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ring.data(), d->exp_ring.p, 4 * ring.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(meta.data(), d->exp_meta.p, 4 * meta.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(vv.data(), d->exp_v.p, 8 * vv.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    delete cell;
+    return c->hip(e, "download");
+  }
+  cell->original = (int64_t)index;
+  const int vc = meta[0];
+  for (uint32_t f = 0; f < fc; ++f) {
+    cell->nbr.push_back(nb[f]);
+    cell->area.push_back(ar[f]);
+    std::vector<double> r;
+    for (int k = 0; k < 64 && ring[f * 64 + k] >= 0; ++k) {
+      const int v = ring[f * 64 + k];
+      r.push_back(vv[3 * v]);
+      r.push_back(vv[3 * v + 1]);
+      r.push_back(vv[3 * v + 2]);
+    }
+    cell->ring.push_back(std::move(r));
+  }
+  for (int v = 0; v < vc && v < 256; ++v)
+    if (meta[2 + v]) {
+      cell->verts.push_back(vv[3 * v]);
+      cell->verts.push_back(vv[3 * v + 1]);
+      cell->verts.push_back(vv[3 * v + 2]);
+    }
+  *out = cell;
+  return TESS_OK;
+}
+
+void tess_cell_destroy(tess_cell* c) { delete c; }
+int64_t tess_cell_original_index(const tess_cell* c) { return c ? c->original : -1; }
+double tess_cell_volume(const tess_cell* c) { return c ? c->volume : std::nan(""); }
+size_t tess_cell_neighbors(const tess_cell* c, int64_t* out, size_t cap) {
+  if (!c) return 0;
+  for (size_t i = 0; i < c->nbr.size() && i < cap && out; ++i) out[i] = c->nbr[i];
+  return c->nbr.size();
+}
+size_t tess_cell_face_count(const tess_cell* c) { return c ? c->nbr.size() : 0; }
+tess_status tess_cell_face(const tess_cell* c, size_t f, double* area, int64_t* neighbor) {
+  if (!c || f >= c->nbr.size()) return TESS_E_INVALID;
+  if (area) *area = c->area[f];
+  if (neighbor) *neighbor = c->nbr[f];
+  return TESS_OK;
+}
+size_t tess_cell_face_vertices(const tess_cell* c, size_t f, double* xyz, size_t cap) {
+  if (!c || f >= c->ring.size()) return 0;
+  const size_t k = c->ring[f].size() / 3;
+  for (size_t i = 0; i < 3 * k && i < 3 * cap && xyz; ++i) xyz[i] = c->ring[f][i];
+  return k;
+}
+size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap) {
+  if (!c) return 0;
+  const size_t k = c->verts.size() / 3;
+  for (size_t i = 0; i < 3 * k && i < 3 * cap && xyz; ++i) xyz[i] = c->verts[i];
+  return k;
+}
+
+tess_status tess_diag_arith(tess_ctx* ctx, const double* a, const double* b, size_t n, double* sq, double* rc,
+                            double* dv) {
+  if (!ctx || !a || !b || !sq || !rc || !dv || n == 0) return TESS_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (ctx->diag.alloc(5 * n) != hipSuccess) return ctx->fail(TESS_E_NOMEM, "device allocation failed");
+  double* p = ctx->diag.p;
+  hipStream_t s = ctx->stream;
+  tess_status st;
+  if ((st = ctx->hip(hipMemcpyAsync(p, a, 8 * n, hipMemcpyHostToDevice, s), "upload")) != TESS_OK) return st;
+  if ((st = ctx->hip(hipMemcpyAsync(p + n, b, 8 * n, hipMemcpyHostToDevice, s), "upload")) != TESS_OK) return st;
+  if ((st = ctx->hip(launch_diag_arith(p, p + n, (int)n, p + 2 * n, p + 3 * n, p + 4 * n, s), "arith")) != TESS_OK)
+    return st;
+  if ((st = ctx->hip(hipMemcpyAsync(sq, p + 2 * n, 8 * n, hipMemcpyDeviceToHost, s), "download")) != TESS_OK)
+    return st;
+  if ((st = ctx->hip(hipMemcpyAsync(rc, p + 3 * n, 8 * n, hipMemcpyDeviceToHost, s), "download")) != TESS_OK)
+    return st;
+  if ((st = ctx->hip(hipMemcpyAsync(dv, p + 4 * n, 8 * n, hipMemcpyDeviceToHost, s), "download")) != TESS_OK)
+    return st;
+  return ctx->hip(hipStreamSynchronize(s), "arith");
+}
+
+}  // extern "C"

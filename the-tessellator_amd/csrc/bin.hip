@@ -1,0 +1,431 @@
+// bin.hip -- uniform-grid counting sort on the device (celery.rs:231-414).
+//
+//   k_soa        AoS xyz -> SoA x, y, z
+//   k_bounds_*   bounding box of the points (celery.rs:81-125) and the grid's
+//                cell sizes / inverse sizes (celery.rs:153-181)
+//   k_bin_count  cell id per point (celery.rs:269-325) and a histogram whose
+//                atomics are aggregated per run of equal ids in a wavefront
+//   k_scan_*     exclusive prefix scan of the histogram = delimiters
+//                (celery.rs:372-414)
+//   k_scatter    points to their cell's range (atomic cursor)
+//   k_sort_cells ascending point index inside each cell (stable order, the
+//                canonical form of celery.rs:357-369)
+//   k_gather     positions in grid order (coalesced candidate loads)
+//   k_compact    per-cell staged faces -> CSR
+// All passes are HBM-bound streams over the points or the cells.
+#include "launch.h"
+#include "tess_internal.h"
+
+namespace tess {
+
+namespace {
+
+__device__ __forceinline__ int axis_index(double v, double vmin, double vmax, double inv, int cpd) {
+  if (v >= vmax) return cpd - 1;
+  double t = (v - vmin) * inv;
+  if (!(t > 0.0)) return 0;
+  if (t >= 2147483647.0) return cpd - 1;
+  int i = (int)t;
+  return i < cpd - 1 ? i : cpd - 1;
+}
+
+}  // namespace
+
+__global__ void k_soa(const double* __restrict__ xyz, int n, double* __restrict__ x,
+                      double* __restrict__ y, double* __restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = xyz[3 * (size_t)i];
+  y[i] = xyz[3 * (size_t)i + 1];
+  z[i] = xyz[3 * (size_t)i + 2];
+}
+
+// Per-block min/max of the three axes -> part[block][6].
+__global__ __launch_bounds__(256) void k_bounds_partial(const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z, int n,
+                                                        double* __restrict__ part) {
+  __shared__ double sm[6][256];
+  const int t = threadIdx.x;
+  double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+  for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) {
+    const double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = v[a] < lo[a] ? v[a] : lo[a];
+      hi[a] = v[a] > hi[a] ? v[a] : hi[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    sm[a][t] = lo[a];
+    sm[3 + a][t] = hi[a];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        sm[a][t] = sm[a][t + s] < sm[a][t] ? sm[a][t + s] : sm[a][t];
+        sm[3 + a][t] = sm[3 + a][t + s] > sm[3 + a][t] ? sm[3 + a][t + s] : sm[3 + a][t];
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 6) part[blockIdx.x * 6 + t] = sm[t][0];
+}
+
+// Reduce the partials; cell sizes as celery.rs:164-170: (max-min)/cpd and cpd/(max-min).
+__global__ void k_bounds_final(const double* __restrict__ part, int nparts, int cpd,
+                               GridDev* __restrict__ g) {
+  if (threadIdx.x != 0) return;
+  double lo[3] = {part[0], part[1], part[2]}, hi[3] = {part[3], part[4], part[5]};
+  for (int b = 1; b < nparts; ++b)
+    for (int a = 0; a < 3; ++a) {
+      const double l = part[6 * b + a], h = part[6 * b + 3 + a];
+      lo[a] = l < lo[a] ? l : lo[a];
+      hi[a] = h > hi[a] ? h : hi[a];
+    }
+  const double c = (double)cpd;
+  g->xmin = lo[0];
+  g->xmax = hi[0];
+  g->ymin = lo[1];
+  g->ymax = hi[1];
+  g->zmin = lo[2];
+  g->zmax = hi[2];
+  g->sx = (hi[0] - lo[0]) / c;
+  g->sy = (hi[1] - lo[1]) / c;
+  g->sz = (hi[2] - lo[2]) / c;
+  g->ix = c / (hi[0] - lo[0]);
+  g->iy = c / (hi[1] - lo[1]);
+  g->iz = c / (hi[2] - lo[2]);
+  g->cpd = cpd;
+  g->pad = 0;
+}
+
+// Cell id of every point; histogram with one atomic per run of equal ids
+// among consecutive lanes (spatially ordered inputs collapse to few atomics).
+__global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z, int n,
+                                                   const GridDev* __restrict__ gp,
+                                                   uint32_t* __restrict__ bins,
+                                                   uint32_t* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const GridDev g = *gp;
+  const int lane = __lane_id();
+  uint32_t b = 0xffffffffu;
+  if (i < n) {
+    const int a = axis_index(x[i], g.xmin, g.xmax, g.ix, g.cpd);
+    const int c = axis_index(y[i], g.ymin, g.ymax, g.iy, g.cpd);
+    const int d = axis_index(z[i], g.zmin, g.zmax, g.iz, g.cpd);
+    b = (uint32_t)((a * g.cpd + c) * g.cpd + d);
+    bins[i] = b;
+  }
+  const uint32_t prev = __shfl_up(b, 1, 64);
+  const bool head = (i < n) && (lane == 0 || prev != b);
+  // both ballots are taken by the whole wavefront (a ballot inside the
+  // branch below would only see the head lanes)
+  const uint64_t heads = __ballot(head);
+  const uint64_t act = __ballot(i < n);
+  if (head) {
+    const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int last = 63 - __clzll((long long)act);  // last active lane
+    const int nexth = after ? lane + 1 + (__ffsll((unsigned long long)after) - 1) : last + 1;
+    atomicAdd(&cnt[b], (uint32_t)(nexth - lane));
+  }
+}
+
+// Three-phase exclusive scan of m counts (u32).  Block = 1024 elements.
+__global__ __launch_bounds__(256) void k_scan_block(const uint32_t* __restrict__ in, int m,
+                                                    uint32_t* __restrict__ out,
+                                                    uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sm[1024];
+  __shared__ uint32_t wsum[4];
+  const int t = threadIdx.x, base = blockIdx.x * 1024;
+  uint32_t v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int idx = base + 4 * t + k;
+    v[k] = idx < m ? in[idx] : 0u;
+    s += v[k];
+  }
+  // exclusive scan of per-thread sums across the block
+  const int lane = t & 63, wid = t >> 6;
+  uint32_t incl = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t woff = 0;
+  for (int w = 0; w < wid; ++w) woff += wsum[w];
+  uint32_t run = woff + incl - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int idx = base + 4 * t + k;
+    if (idx < m) out[idx] = run;
+    run += v[k];
+  }
+  if (t == 255) sums[blockIdx.x] = woff + incl;
+  (void)sm;
+}
+
+// Exclusive scan of the block sums in place (one block, sequential chunks).
+__global__ __launch_bounds__(256) void k_scan_sums(uint32_t* __restrict__ sums, int nb,
+                                                   uint32_t* __restrict__ total) {
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t carry;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 256) {
+    const int idx = base + t;
+    const uint32_t v = idx < nb ? sums[idx] : 0u;
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t woff = carry;
+    for (int w = 0; w < wid; ++w) woff += wsum[w];
+    if (idx < nb) sums[idx] = woff + incl - v;
+    __syncthreads();
+    if (t == 255) carry = woff + incl;
+    __syncthreads();
+  }
+  if (t == 0) *total = carry;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, int m, const uint32_t* __restrict__ sums,
+                           const uint32_t* __restrict__ total, uint32_t* __restrict__ last) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] += sums[i >> 10];
+  if (i == 0 && last) *last = *total;
+}
+
+__global__ void k_scatter(const uint32_t* __restrict__ bins, int n, uint32_t* __restrict__ cursor,
+                          uint32_t* __restrict__ sorted) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t pos = atomicAdd(&cursor[bins[i]], 1u);
+  sorted[pos] = (uint32_t)i;
+}
+
+// Ascending point index inside each cell: insertion sort for small cells,
+// one wavefront per cell (rank by comparison) for large ones.
+__global__ void k_sort_cells(const uint32_t* __restrict__ delim, int nbins,
+                             uint32_t* __restrict__ sorted, uint32_t* __restrict__ big,
+                             uint32_t* __restrict__ nbig) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const uint32_t s = delim[b], e = delim[b + 1];
+  if (e - s <= 1) return;
+  if (e - s > 32) {
+    const uint32_t k = atomicAdd(nbig, 1u);
+    big[k] = (uint32_t)b;
+    return;
+  }
+  for (uint32_t i = s + 1; i < e; ++i) {
+    const uint32_t v = sorted[i];
+    uint32_t j = i;
+    while (j > s && sorted[j - 1] > v) {
+      sorted[j] = sorted[j - 1];
+      --j;
+    }
+    sorted[j] = v;
+  }
+}
+
+// Large cells: rank of each element = number of smaller ids in the cell
+// (ids are distinct), written into a scratch copy then back.
+__global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ delim,
+                                                  const uint32_t* __restrict__ big,
+                                                  const uint32_t* __restrict__ nbig,
+                                                  uint32_t* __restrict__ sorted,
+                                                  uint32_t* __restrict__ scratch) {
+  const uint32_t nb = *nbig;
+  for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
+    const uint32_t b = big[q];
+    const uint32_t s = delim[b], e = delim[b + 1];
+    for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+      const uint32_t v = sorted[i];
+      uint32_t r = 0;
+      for (uint32_t j = s; j < e; ++j) r += sorted[j] < v;
+      scratch[s + r] = v;
+    }
+    __syncthreads();
+    for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) sorted[i] = scratch[i];
+    __syncthreads();
+  }
+}
+
+__global__ void k_gather(const uint32_t* __restrict__ sorted, int n, const double* __restrict__ x,
+                         const double* __restrict__ y, const double* __restrict__ z,
+                         double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = sorted[k];
+  sx[k] = x[i];
+  sy[k] = y[i];
+  sz[k] = z[i];
+}
+
+// Exclusive scan of face counts into u64 offsets: block scan in u64.
+__global__ __launch_bounds__(256) void k_scan64_block(const uint32_t* __restrict__ in, int m,
+                                                      uint64_t* __restrict__ out,
+                                                      uint64_t* __restrict__ sums) {
+  __shared__ uint64_t wsum[4];
+  const int t = threadIdx.x, base = blockIdx.x * 1024, lane = t & 63, wid = t >> 6;
+  uint64_t v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int idx = base + 4 * t + k;
+    v[k] = idx < m ? (uint64_t)in[idx] : 0ull;
+    s += v[k];
+  }
+  uint64_t incl = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint64_t u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint64_t woff = 0;
+  for (int w = 0; w < wid; ++w) woff += wsum[w];
+  uint64_t run = woff + incl - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int idx = base + 4 * t + k;
+    if (idx < m) out[idx] = run;
+    run += v[k];
+  }
+  if (t == 255) sums[blockIdx.x] = woff + incl;
+}
+
+__global__ __launch_bounds__(256) void k_scan64_sums(uint64_t* __restrict__ sums, int nb) {
+  __shared__ uint64_t wsum[4];
+  __shared__ uint64_t carry;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 256) {
+    const int idx = base + t;
+    const uint64_t v = idx < nb ? sums[idx] : 0ull;
+    uint64_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint64_t u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint64_t woff = carry;
+    for (int w = 0; w < wid; ++w) woff += wsum[w];
+    if (idx < nb) sums[idx] = woff + incl - v;
+    __syncthreads();
+    if (t == 255) carry = woff + incl;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scan64_add(uint64_t* __restrict__ out, int m, const uint64_t* __restrict__ sums,
+                             const uint32_t* __restrict__ in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] += sums[i >> 10];
+  if (i == m - 1) out[m] = out[i] + in[i];
+}
+
+// Staged faces -> CSR: 16 lanes per cell.
+__global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ fcount,
+                                                 const uint64_t* __restrict__ foff, int n,
+                                                 const int32_t* __restrict__ snbr,
+                                                 const double* __restrict__ sarea,
+                                                 int32_t* __restrict__ nbr, double* __restrict__ area) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cell = g >> 4, sub = g & 15;
+  if (cell >= n) return;
+  const uint32_t fc = fcount[cell];
+  const uint64_t o = foff[cell];
+  for (uint32_t f = sub; f < fc; f += 16) {
+    nbr[o + f] = snbr[(size_t)cell * kFaceStage + f];
+    area[o + f] = sarea[(size_t)cell * kFaceStage + f];
+  }
+}
+
+}  // namespace tess
+
+namespace tess {
+
+namespace {
+__global__ void k_invert(const uint32_t* __restrict__ sorted, int n, uint32_t* __restrict__ inv) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) inv[sorted[k]] = (uint32_t)k;
+}
+inline int blocks(long long n, int t) { return (int)((n + t - 1) / t); }
+}  // namespace
+
+hipError_t launch_soa(const double* xyz, int n, double* x, double* y, double* z, hipStream_t s) {
+  hipLaunchKernelGGL(k_soa, dim3(blocks(n, 256)), dim3(256), 0, s, xyz, n, x, y, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_bounds(const double* x, const double* y, const double* z, int n, int cpd,
+                         double* part, int nparts, GridDev* g, hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds_partial, dim3(nparts), dim3(256), 0, s, x, y, z, n, part);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, part, nparts, cpd, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
+                                const GridDev* g, int nbins, const SortBufs& b, hipStream_t s) {
+  hipError_t e;
+  if ((e = hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * (size_t)nbins, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(b.misc, 0, sizeof(uint32_t) * 4, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt);
+  const int nsb = blocks(nbins, 1024);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, b.cnt, nbins, b.delim, b.sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, b.sums, nsb, b.misc);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sums,
+                     b.misc, b.delim + nbins);
+  if ((e = hipMemcpyAsync(b.cursor, b.delim, sizeof(uint32_t) * (size_t)nbins, hipMemcpyDeviceToDevice,
+                          s)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, b.bins, n, b.cursor, b.sorted);
+  hipLaunchKernelGGL(k_sort_cells, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sorted,
+                     b.big, b.misc + 1);
+  hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(256), 0, s, b.delim, b.big, b.misc + 1, b.sorted,
+                     b.scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const uint32_t* sorted, int n, const double* x, const double* y,
+                         const double* z, double* sx, double* sy, double* sz, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, s, sorted, n, x, y, z, sx, sy, sz);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_offsets(const uint32_t* fcount, int n, uint64_t* foff, uint64_t* sums,
+                               hipStream_t s) {
+  const int nsb = blocks(n, 1024);
+  hipLaunchKernelGGL(k_scan64_block, dim3(nsb), dim3(256), 0, s, fcount, n, foff, sums);
+  hipLaunchKernelGGL(k_scan64_sums, dim3(1), dim3(256), 0, s, sums, nsb);
+  hipLaunchKernelGGL(k_scan64_add, dim3(blocks(n, 256)), dim3(256), 0, s, foff, n, sums, fcount);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(const uint32_t* fcount, const uint64_t* foff, int n, const int32_t* snbr,
+                          const double* sarea, int32_t* nbr, double* area, hipStream_t s) {
+  hipLaunchKernelGGL(k_compact, dim3(blocks((long long)n * 16, 256)), dim3(256), 0, s, fcount, foff, n,
+                     snbr, sarea, nbr, area);
+  return hipGetLastError();
+}
+
+hipError_t launch_invert(const uint32_t* sorted, int n, uint32_t* inv, hipStream_t s) {
+  hipLaunchKernelGGL(k_invert, dim3(blocks(n, 256)), dim3(256), 0, s, sorted, n, inv);
+  return hipGetLastError();
+}
+
+}  // namespace tess

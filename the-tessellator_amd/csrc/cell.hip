@@ -1,0 +1,834 @@
+// cell.hip -- one Voronoi cell per wavefront (gfx950, wave64).
+//
+// Per cell (Cell::compute_voronoi_cell, interface.rs:257-313):
+//   * the container box, translated by -site, is the starting polyhedron
+//     (polyhedron.rs:268-392, :859-867), held in LDS as vertices + half-edge
+//     pairs (flip(h) = h ^ 1) + faces;
+//   * candidates arrive in the reference's shell order (celery.rs:418-679,
+//     :971-1018): 64 table entries per batch, their grid cells' points
+//     gathered 64 per chunk (lane = candidate), bisector planes built
+//     lane-parallel (vector3.rs:223-239);
+//   * candidates are then applied in order: lanes classify vertices
+//     (vector3.rs:170-193, tolerance 1e-12), a ballot decides whether the
+//     cell changes, and a lane-parallel clip rewires the half-edges: lanes
+//     find the Outside runs of every face, one lane per run places the exit
+//     point intersection(outside, inside) (vector3.rs:213-219) and the bridge
+//     half-edge, and the new face (tagged with the point id) is linked
+//     through an exit-point map;
+//   * the security-radius stop: a shell entry whose key exceeds
+//     (4 R^2)(1 + 1e-9), R = max |vertex|, and every candidate with |p|^2
+//     above the same bound, cannot cut the cell, so skipping them leaves the
+//     result bit-identical to streaming every point;
+//   * outputs (polyhedron.rs:776-855, interface.rs:408-410): fan-triangulated
+//     weighted normals anchored at each face's lexicographically smallest
+//     vertex, volume summed over faces in creation order, areas, neighbours
+//     sorted ascending -- all written coalesced per cell.
+// All f64 arithmetic is compiled with -ffp-contract=off so every operation
+// is one IEEE-rounded operation in the same order as the CPU restatement.
+#include <math.h>
+
+#include "launch.h"
+#include "tess_internal.h"
+
+namespace tess {
+
+namespace {
+
+constexpr double kTol = 1e-12;  // polyhedron.rs:221-223
+enum : uint8_t { C_OUT = 0, C_INC = 1, C_IN = 2 };
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
+__device__ __forceinline__ int ctz64(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+__device__ __forceinline__ int mbcnt(uint64_t m, int lane) {
+  return __popcll(m & ((1ull << lane) - 1ull));
+}
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rl_d(double v, int l) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+  int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double t = __shfl_xor(v, o, 64);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Position of the k-th (0-based) set bit of w; w must have more than k bits.
+__device__ __forceinline__ int select_bit(uint64_t w, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    int c = __popcll(w & ((1ull << sh) - 1ull));
+    if (k >= c) {
+      k -= c;
+      w >>= sh;
+      pos += sh;
+    }
+  }
+  return pos;
+}
+template <int W>
+__device__ __forceinline__ int select_multi(const uint64_t (&m)[W], int k) {
+#pragma unroll
+  for (int r = 0; r < W; ++r) {
+    int c = __popcll(m[r]);
+    if (k < c) return 64 * r + select_bit(m[r], k);
+    k -= c;
+  }
+  return -1;
+}
+template <int W>
+__device__ __forceinline__ int popc_multi(const uint64_t (&m)[W]) {
+  int c = 0;
+#pragma unroll
+  for (int r = 0; r < W; ++r) c += __popcll(m[r]);
+  return c;
+}
+// Lowest k set bits of m (k <= popcount(m)).
+template <int W>
+__device__ __forceinline__ void lowest_bits(const uint64_t (&m)[W], int k, uint64_t (&out)[W]) {
+#pragma unroll
+  for (int r = 0; r < W; ++r) {
+    int c = __popcll(m[r]);
+    if (k >= c) {
+      out[r] = m[r];
+      k -= c;
+    } else if (k == 0) {
+      out[r] = 0;
+    } else {
+      int pos = select_bit(m[r], k);  // first bit not taken
+      out[r] = m[r] & ((1ull << pos) - 1ull);
+      k = 0;
+    }
+  }
+}
+
+__device__ __forceinline__ int axis_index(double v, double vmin, double vmax, double inv, int cpd) {
+  if (v >= vmax) return cpd - 1;  // celery.rs:269-282
+  double t = (v - vmin) * inv;
+  if (!(t > 0.0)) return 0;       // Rust `as usize` saturates, NaN -> 0
+  if (t >= 2147483647.0) return cpd - 1;
+  int i = (int)t;
+  return i < cpd - 1 ? i : cpd - 1;
+}
+
+__device__ __forceinline__ double stop_thr(double r2) { return (4.0 * r2) * (1.0 + 1e-9); }
+
+__device__ __forceinline__ bool lex_less(double ax, double ay, double az, double bx, double by, double bz) {
+  if (ax != bx) return ax < bx;
+  if (ay != by) return ay < by;
+  return az < bz;
+}
+
+// Container cube in half-edge pairs.  Vertices FDL FDR FUR FUL BDL BDR BUR
+// BUL, faces F(-y) R(+x) B(+y) L(-x) U(+z) D(-z) (polyhedron.rs:73-392,
+// with DR on face D).
+__constant__ uint8_t kCubeTgt[24] = {3, 2, 0, 3, 1, 0, 2, 1, 2, 6, 5, 1, 6, 5, 6, 7, 4, 5, 7, 4, 7, 3, 0, 4};
+__constant__ uint16_t kCubeNext[24] = {2,  9,  4, 20, 6,  23, 0,  10, 7,  15, 12, 5,
+                                       8, 16, 13, 21, 18, 11, 14, 22, 19, 1,  3,  17};
+__constant__ uint8_t kCubeFace[24] = {0, 4, 0, 3, 0, 5, 0, 1, 1, 4, 1, 5, 1, 2, 2, 4, 2, 5, 2, 3, 3, 4, 3, 5};
+__constant__ uint16_t kCubeStart[6] = {0, 8, 14, 20, 1, 5};
+__constant__ int32_t kWallTag[6] = {-3, -2, -4, -1, -6, -5};
+
+template <int VPL, int PPL, int FPL>
+struct Lds {
+  static constexpr int VC = 64 * VPL, PC = 64 * PPL, HC = 2 * PC, FC = 64 * FPL;
+  double vx[VC], vy[VC], vz[VC], vdot[VC];
+  double fcontrib[FC];
+  int32_t f_tag[FC];
+  uint32_t bstart[64];
+  uint32_t bpref[64];
+  uint16_t he_next[HC];
+  uint16_t f_start[FC];
+  uint16_t f_seq[FC];
+  uint16_t exit_of[VC];
+  uint16_t run_a[64];
+  uint8_t he_tgt[HC];
+  uint8_t he_face[HC];
+  uint8_t newv_of[HC];
+  uint8_t vcls[VC];
+  uint8_t fmark[FC];
+};
+
+enum { CUT_OK = 0, CUT_CAPACITY = 1, CUT_DEGENERATE = 2 };
+
+template <int VPL, int PPL, int FPL>
+struct CellState {
+  uint64_t VA[VPL];  // live vertex slots
+  uint64_t PA[PPL];  // live half-edge pairs
+  uint64_t FA[FPL];  // live faces
+  int seq;           // next face creation number
+  double rx[VPL], ry[VPL], rz[VPL];  // this lane's vertex slots (register copy)
+};
+
+// Apply one bisector that puts at least one vertex Outside.  OM: Outside
+// vertices; S.vcls / S.vdot hold every live vertex's class and n.v.
+template <int VPL, int PPL, int FPL>
+__device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const uint64_t (&OM)[VPL],
+                    double off, int32_t tag, int lane, int* newv) {
+  constexpr int HC = Lds<VPL, PPL, FPL>::HC;
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) S.exit_of[lane + 64 * r] = 0xffff;
+
+  // 1. scan pairs: Outside-run entries (P -> O, P not Outside) and pairs to
+  //    delete (an Outside end whose other end is not Inside).
+  uint64_t PD[PPL];
+  int nruns = 0;
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    const int p = lane + 64 * r;
+    const bool al = (C.PA[r] >> lane) & 1ull;
+    bool ent = false, del = false;
+    int eh = 0;
+    if (al) {
+      const int t0 = S.he_tgt[2 * p], t1 = S.he_tgt[2 * p + 1];
+      const uint8_t c0 = S.vcls[t0], c1 = S.vcls[t1];
+      if (c0 == C_OUT && c1 != C_OUT) {
+        ent = true;
+        eh = 2 * p;
+      } else if (c1 == C_OUT && c0 != C_OUT) {
+        ent = true;
+        eh = 2 * p + 1;
+      }
+      del = (c0 == C_OUT && c1 != C_IN) || (c1 == C_OUT && c0 != C_IN);
+    }
+    PD[r] = ballot(del);
+    const uint64_t em = ballot(ent);
+    const int pos = nruns + mbcnt(em, lane);
+    if (ent && pos < 64) S.run_a[pos] = (uint16_t)eh;
+    nruns += popc(em);
+  }
+  if (nruns > 64) return CUT_CAPACITY;
+  if (nruns < 1) return CUT_DEGENERATE;
+  __syncthreads();
+
+  // 2. one lane per run: walk to the exit, classify the run's ends.
+  const bool isrun = lane < nruns;
+  int a = 0, b = 0, P = 0, Q = 0, Ok = 0, nb = 0, pred = 0, fa = 0;
+  uint8_t cP = C_IN, cQ = C_IN;
+  bool one = false, two = false, bad = false;
+  if (isrun) {
+    a = S.run_a[lane];
+    P = S.he_tgt[a ^ 1];
+    cP = S.vcls[P];
+    int x = S.he_next[a], g = 0;
+    while (S.vcls[S.he_tgt[x]] == C_OUT) {
+      x = S.he_next[x];
+      if (++g > HC) {
+        bad = true;
+        break;
+      }
+    }
+    b = x;
+    Q = S.he_tgt[b];
+    cQ = S.vcls[Q];
+    Ok = S.he_tgt[b ^ 1];
+    nb = S.he_next[b];
+    fa = S.he_face[a];
+    if (cP != C_IN && cQ != C_IN) {
+      if (nb == a)
+        one = true;  // face reduced to the single vertex P == Q
+      else if (S.he_tgt[nb] == P && S.he_next[nb] == a)
+        two = true;  // face reduced to the edge Q -> P
+    }
+    if (!one && !two && cP != C_IN) {
+      int u = (cQ == C_IN) ? b : nb;
+      int g2 = 0;
+      while (S.he_next[u] != a) {
+        u = S.he_next[u];
+        if (++g2 > HC) {
+          bad = true;
+          break;
+        }
+      }
+      pred = u;
+    }
+  }
+  if (ballot(bad)) return CUT_DEGENERATE;
+  const bool needX = isrun && cQ == C_IN;
+  const bool normal = isrun && !one && !two;
+
+  // 3. allocate vertex slots (reusing Outside slots), pairs, the new face.
+  uint64_t vkeep[VPL], vfree[VPL], pkeep[PPL], pfree[PPL];
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) {
+    vkeep[r] = C.VA[r] & ~OM[r];
+    vfree[r] = ~vkeep[r];
+  }
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    pkeep[r] = C.PA[r] & ~PD[r];
+    pfree[r] = ~pkeep[r];
+  }
+  const uint64_t xm = ballot(needX), bm = ballot(normal);
+  const int nX = popc(xm), nB = popc(bm);
+  if (nX > popc_multi(vfree) || nB > popc_multi(pfree)) return CUT_CAPACITY;
+  int nf = -1;
+#pragma unroll
+  for (int r = FPL - 1; r >= 0; --r)
+    if (~C.FA[r]) nf = 64 * r + ctz64(~C.FA[r]);
+  if (nf < 0) return CUT_CAPACITY;
+  const int vslot = needX ? select_multi(vfree, mbcnt(xm, lane)) : 0;
+  const int pslot = normal ? select_multi(pfree, mbcnt(bm, lane)) : 0;
+
+  // 4. exit points: intersection(outside, inside), vector3.rs:213-219
+  double X0 = 0, X1 = 0, X2 = 0;
+  if (needX) {
+    const double dO = S.vdot[Ok], dQ = S.vdot[Q];
+    const double t = (off - dO) / (dQ - dO);
+    const double ox = S.vx[Ok], oy = S.vy[Ok], oz = S.vz[Ok];
+    X0 = ox + (S.vx[Q] - ox) * t;
+    X1 = oy + (S.vy[Q] - oy) * t;
+    X2 = oz + (S.vz[Q] - oz) * t;
+  }
+  __syncthreads();
+  if (needX) {
+    S.vx[vslot] = X0;
+    S.vy[vslot] = X1;
+    S.vz[vslot] = X2;
+    S.newv_of[b] = (uint8_t)vslot;
+  }
+  __syncthreads();
+  const int Xid = needX ? vslot : Q;
+  const int Eid = isrun ? ((cP == C_IN) ? (int)S.newv_of[a ^ 1] : P) : 0;
+
+  // 5. rewire: bridge on the run's face, its flip on the new face.
+  int nfh = -1;
+  if (normal) {
+    const int br = 2 * pslot, brf = br + 1;
+    S.he_tgt[br] = (uint8_t)Xid;
+    S.he_face[br] = (uint8_t)fa;
+    S.he_next[br] = (uint16_t)(needX ? b : nb);
+    S.he_tgt[brf] = (uint8_t)Eid;
+    S.he_face[brf] = (uint8_t)nf;
+    if (cP == C_IN) {
+      S.he_tgt[a] = (uint8_t)Eid;
+      S.he_next[a] = (uint16_t)br;
+    } else {
+      S.he_next[pred] = (uint16_t)br;
+    }
+    S.f_start[fa] = (uint16_t)br;
+    S.exit_of[Xid] = (uint16_t)brf;
+    nfh = brf;
+  } else if (two) {
+    S.he_face[nb] = (uint8_t)nf;
+    S.exit_of[Q] = (uint16_t)nb;
+    nfh = nb;
+  }
+  __syncthreads();
+  bool lbad = false;
+  if (nfh >= 0) {
+    const int nx = S.exit_of[Eid];
+    if (nx == 0xffff)
+      lbad = true;
+    else
+      S.he_next[nfh] = (uint16_t)nx;
+  }
+  if (ballot(lbad)) return CUT_DEGENERATE;
+  const uint64_t hm = ballot(nfh >= 0);
+  if (!hm) return CUT_DEGENERATE;
+  const int nfstart = rl_i(nfh, ctz64(hm));
+
+  // 6. live sets
+  uint64_t vnew[VPL], pnew[PPL];
+  lowest_bits(vfree, nX, vnew);
+  lowest_bits(pfree, nB, pnew);
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) C.VA[r] = vkeep[r] | vnew[r];
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) C.PA[r] = pkeep[r] | pnew[r];
+#pragma unroll
+  for (int r = 0; r < FPL; ++r) S.fmark[lane + 64 * r] = 0;
+  if (lane == 0) {
+    S.f_tag[nf] = tag;
+    S.f_seq[nf] = (uint16_t)C.seq;
+    S.f_start[nf] = (uint16_t)nfstart;
+  }
+  C.seq++;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    const int p = lane + 64 * r;
+    if ((C.PA[r] >> lane) & 1ull) {
+      S.fmark[S.he_face[2 * p]] = 1;
+      S.fmark[S.he_face[2 * p + 1]] = 1;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < FPL; ++r) C.FA[r] = ballot(S.fmark[lane + 64 * r] != 0);
+  // refresh this lane's vertex registers
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) {
+    const int v = lane + 64 * r;
+    if ((C.VA[r] >> lane) & 1ull) {
+      C.rx[r] = S.vx[v];
+      C.ry[r] = S.vy[v];
+      C.rz[r] = S.vz[v];
+    }
+  }
+  *newv = nX;
+  return CUT_OK;
+}
+
+template <int VPL, int PPL, int FPL>
+__device__ double cell_r2(const CellState<VPL, PPL, FPL>& C, int lane) {
+  double m = 0.0;
+#pragma unroll
+  for (int r = 0; r < VPL; ++r)
+    if ((C.VA[r] >> lane) & 1ull) {
+      double d = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
+      m = d > m ? d : m;
+    }
+  return wave_max(m);
+}
+
+}  // namespace
+
+template <int VPL, int PPL, int FPL>
+__global__ __launch_bounds__(64) void k_cells(CellArgs A) {
+  using L = Lds<VPL, PPL, FPL>;
+  constexpr int FC = L::FC;
+  __shared__ L S;
+  CellState<VPL, PPL, FPL> C;
+  const int lane = threadIdx.x;
+
+  int w = blockIdx.x;
+  if (A.xcd_remap) {
+    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  if (w >= A.nwork) return;
+  const int gpos = A.work ? (int)A.work[w] : w;
+  const uint32_t self = A.sorted[gpos];
+  const double sx = A.spx[gpos], sy = A.spy[gpos], sz = A.spz[gpos];
+  const GridDev G = *A.grid;
+  const int cpd = G.cpd;
+  const int ix = axis_index(sx, G.xmin, G.xmax, G.ix, cpd);
+  const int iy = axis_index(sy, G.ymin, G.ymax, G.iy, cpd);
+  const int iz = axis_index(sz, G.zmin, G.zmax, G.iz, cpd);
+
+  // ---- container cube translated by -site (polyhedron.rs:226-392, :859-867)
+  {
+    const double x0 = A.lo[0] + (-sx), x1 = A.hi[0] + (-sx);
+    const double y0 = A.lo[1] + (-sy), y1 = A.hi[1] + (-sy);
+    const double z0 = A.lo[2] + (-sz), z1 = A.hi[2] + (-sz);
+#pragma unroll
+    for (int r = 0; r < VPL; ++r) {
+      const int v = lane + 64 * r;
+      if (v < 8) {
+        const double vx = (v == 1 || v == 2 || v == 5 || v == 6) ? x1 : x0;
+        const double vy = (v >= 4) ? y1 : y0;
+        const double vz = (v == 2 || v == 3 || v == 6 || v == 7) ? z1 : z0;
+        S.vx[v] = vx;
+        S.vy[v] = vy;
+        S.vz[v] = vz;
+        C.rx[r] = vx;
+        C.ry[r] = vy;
+        C.rz[r] = vz;
+      }
+    }
+    if (lane < 24) {
+      S.he_tgt[lane] = kCubeTgt[lane];
+      S.he_next[lane] = kCubeNext[lane];
+      S.he_face[lane] = kCubeFace[lane];
+    }
+    if (lane < 6) {
+      S.f_tag[lane] = kWallTag[lane];
+      S.f_start[lane] = kCubeStart[lane];
+      S.f_seq[lane] = (uint16_t)lane;
+    }
+#pragma unroll
+    for (int r = 0; r < VPL; ++r) C.VA[r] = 0;
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) C.PA[r] = 0;
+#pragma unroll
+    for (int r = 0; r < FPL; ++r) C.FA[r] = 0;
+    C.VA[0] = 0xffull;
+    C.PA[0] = 0xfffull;
+    C.FA[0] = 0x3full;
+    C.seq = 6;
+  }
+  __syncthreads();
+  double r2 = cell_r2(C, lane);
+
+  uint32_t streamed = 0, classified = 0, ncuts = 0, status = 0;
+  uint64_t flops = 24 + 40;
+  int maxv = 8;
+  int e_lastcut = -1;
+  int e_stop_final = 0;
+  bool done = false;
+  const double rlim = A.radius_limit;
+
+  for (int ebase = 0; !done; ebase += 64) {
+    if (ebase >= A.tlen) {
+      // table exhausted: every later entry has key >= kcut
+      if (!(A.kcut > stop_thr(r2))) status |= TESS_CELL_TABLE;
+      e_stop_final = A.tlen;
+      break;
+    }
+    const int e = ebase + lane;
+    const bool have = e < A.tlen;
+    double key = 0.0;
+    uint32_t bst = 0;
+    int cnt = 0;
+    if (have) {
+      key = A.tkey[e];
+      int ox, oy, oz;
+      unpack_off(A.toff[e], ox, oy, oz);
+      const int bx = ix + ox, by = iy + oy, bz = iz + oz;
+      if (bx >= 0 && bx < cpd && by >= 0 && by < cpd && bz >= 0 && bz < cpd) {
+        const int bin = (bx * cpd + by) * cpd + bz;
+        bst = A.delim[bin];
+        cnt = (int)(A.delim[bin + 1] - bst);
+      }
+    }
+    const uint64_t sm = ballot(have && (key > stop_thr(r2) || key > rlim));
+    const uint64_t em = ballot(!have);
+    const int nE = sm ? ctz64(sm) : 64;
+    const int nEnd = em ? ctz64(em) : 64;
+    const int lim = nE < nEnd ? nE : nEnd;
+    int e_stop = ebase + lim;  // entries [ebase, e_stop) are reached
+    bool stop_here = (sm != 0) && nE <= nEnd;
+    const int c = lane < lim ? cnt : 0;
+    const int incl = wave_incl_scan(c, lane);
+    const int total = rl_i(incl, 63);
+    S.bpref[lane] = (uint32_t)(incl - c);
+    S.bstart[lane] = bst;
+    __syncthreads();
+
+    for (int base = 0; base < total && !done; base += 64) {
+      const int k = base + lane;
+      bool valid = false;
+      int ek = 0;
+      uint32_t jidx = 0;
+      double px = 0, py = 0, pz = 0, d2 = 0;
+      if (k < total) {
+        int lo = 0, hi = lim - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if ((int)S.bpref[mid] <= k)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        ek = lo;
+        const uint32_t s = S.bstart[lo] + (uint32_t)(k - (int)S.bpref[lo]);
+        jidx = A.sorted[s];
+        valid = (jidx != self) && (A.target_group < 0 || A.groups[jidx] == A.target_group);
+        px = A.spx[s] - sx;
+        py = A.spy[s] - sy;
+        pz = A.spz[s] - sz;
+        d2 = (px * px + py * py) + pz * pz;
+      }
+      const int eg = ebase + ek;
+      const bool keep = valid && eg < e_stop && !(d2 > stop_thr(r2));
+      double nx = 0, ny = 0, nz = 0, off = 0;
+      if (keep) {  // halfway_from_origin_to, vector3.rs:223-239
+        const double mag = sqrt(d2);
+        const double inv = 1.0 / mag;
+        nx = px * inv;
+        ny = py * inv;
+        nz = pz * inv;
+        off = (nx * (px * 0.5) + ny * (py * 0.5)) + nz * (pz * 0.5);
+      }
+      uint64_t km = ballot(keep);
+      while (km) {
+        const int cl = ctz64(km);
+        km &= km - 1;
+        const int ec = ebase + rl_i(ek, cl);
+        if (ec >= e_stop) break;
+        const double cd2 = rl_d(d2, cl);
+        if (cd2 > stop_thr(r2)) continue;
+        const double cnx = rl_d(nx, cl), cny = rl_d(ny, cl), cnz = rl_d(nz, cl), coff = rl_d(off, cl);
+        int nv = popc_multi(C.VA);
+        classified++;
+        flops += 13 + 6ull * (uint64_t)nv;
+        uint64_t OM[VPL];
+        uint8_t cv[VPL];
+        double dv[VPL];
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < VPL; ++r) {
+          const bool al = (C.VA[r] >> lane) & 1ull;
+          uint8_t cc = C_IN;
+          double d = 0.0;
+          if (al) {
+            d = (cnx * C.rx[r] + cny * C.ry[r]) + cnz * C.rz[r];
+            const double s = d - coff;
+            cc = s > kTol ? C_OUT : (s < -kTol ? C_IN : C_INC);
+          }
+          cv[r] = cc;
+          dv[r] = d;
+          OM[r] = ballot(al && cc == C_OUT);
+          any = any || OM[r] != 0;
+        }
+        if (!any) continue;
+#pragma unroll
+        for (int r = 0; r < VPL; ++r)
+          if ((C.VA[r] >> lane) & 1ull) {
+            S.vcls[lane + 64 * r] = cv[r];
+            S.vdot[lane + 64 * r] = dv[r];
+          }
+        __syncthreads();
+        int newv = 0;
+        const int rc = clip(S, C, OM, coff, (int32_t)rl_i((int)jidx, cl), lane, &newv);
+        if (rc != CUT_OK) {
+          status |= (rc == CUT_CAPACITY) ? TESS_CELL_CAPACITY : TESS_CELL_DEGENERATE;
+          done = true;
+          break;
+        }
+        ncuts++;
+        nv = popc_multi(C.VA);
+        if (nv > maxv) maxv = nv;
+        flops += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
+        r2 = cell_r2(C, lane);
+        e_lastcut = ec;
+        const uint64_t sm2 =
+            ballot(lane < lim && ebase + lane > e_lastcut && (key > stop_thr(r2) || key > rlim));
+        if (sm2) {
+          e_stop = ebase + ctz64(sm2);
+          stop_here = true;
+        }
+      }
+      streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
+      if (stop_here && base + 64 < total) {
+        // candidates of the next chunk start at an entry >= e_stop?
+        const int kn = base + 64;
+        int lo = 0, hi = lim - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if ((int)S.bpref[mid] <= kn)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        if (ebase + lo >= e_stop) break;
+      }
+    }
+    __syncthreads();
+    if (stop_here || done) {
+      e_stop_final = e_stop;
+      done = true;
+    }
+  }
+  flops += 8ull * streamed;
+
+  // ---- outputs
+  const uint32_t oi = self;
+  int nfaces = popc_multi(C.FA);
+  const bool ok = (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0;
+  if (ok) {
+    const int nv = popc_multi(C.VA), ne = popc_multi(C.PA);
+    if (nv - ne + nfaces != 2) status |= TESS_CELL_DEGENERATE;
+  }
+  if (nfaces > kFaceStage) status |= TESS_CELL_CAPACITY;
+  uint64_t oflops = 0;
+  if ((status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0) {
+    double area[FPL], contrib[FPL];
+    int32_t tagv[FPL];
+    int seqv[FPL];
+    bool fal[FPL];
+#pragma unroll
+    for (int r = 0; r < FPL; ++r) {
+      const int f = lane + 64 * r;
+      fal[r] = (C.FA[r] >> lane) & 1ull;
+      area[r] = 0;
+      contrib[r] = 0;
+      tagv[r] = 0;
+      seqv[r] = 0;
+      if (fal[r]) {
+        const int st = S.f_start[f];
+        int h = st, ha = st, k = 0;
+        int va = S.he_tgt[st];
+        double ax = S.vx[va], ay = S.vy[va], az = S.vz[va];
+        do {
+          const int v = S.he_tgt[h];
+          const double x = S.vx[v], y = S.vy[v], z = S.vz[v];
+          if (lex_less(x, y, z, ax, ay, az)) {
+            ax = x;
+            ay = y;
+            az = z;
+            ha = h;
+          }
+          h = S.he_next[h];
+          ++k;
+        } while (h != st && k <= 255);
+        // fan from the anchor: sum cross(V[j-1]-A, V[j]-A), j = 2..k-1
+        double wx = 0.0, wy = 0.0, wz = 0.0;
+        h = S.he_next[ha];
+        int v = S.he_tgt[h];
+        double cx = S.vx[v] - ax, cy = S.vy[v] - ay, cz = S.vz[v] - az;
+        for (int j = 2; j < k; ++j) {
+          h = S.he_next[h];
+          v = S.he_tgt[h];
+          const double qx = cx, qy = cy, qz = cz;
+          cx = S.vx[v] - ax;
+          cy = S.vy[v] - ay;
+          cz = S.vz[v] - az;
+          wx = wx + (qy * cz - qz * cy);
+          wy = wy + (qz * cx - qx * cz);
+          wz = wz + (qx * cy - qy * cx);
+        }
+        contrib[r] = (ax * wx + ay * wy) + az * wz;
+        area[r] = 0.5 * sqrt((wx * wx + wy * wy) + wz * wz);
+        tagv[r] = S.f_tag[f];
+        seqv[r] = S.f_seq[f];
+        oflops += 3ull * (k - 1) + 12ull * (k - 2) + 13;
+      }
+    }
+    // rank faces by creation number (volume order) and by tag (output order)
+    int rs[FPL], rt[FPL];
+#pragma unroll
+    for (int r = 0; r < FPL; ++r) {
+      rs[r] = 0;
+      rt[r] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      uint64_t m = C.FA[q];
+      while (m) {
+        const int g = 64 * q + ctz64(m);
+        m &= m - 1;
+        const int gs = S.f_seq[g];
+        const int32_t gt = S.f_tag[g];
+#pragma unroll
+        for (int r = 0; r < FPL; ++r) {
+          rs[r] += (gs < seqv[r]);
+          rt[r] += (gt < tagv[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < FPL; ++r)
+      if (fal[r]) S.fcontrib[rs[r]] = contrib[r];
+    __syncthreads();
+    if (lane == 0) {
+      double vol = 0.0;
+      for (int f = 0; f < nfaces; ++f) vol = vol + S.fcontrib[f];
+      A.volume[oi] = vol / 6.0;
+      A.fcount[oi] = (uint32_t)nfaces;
+    }
+#pragma unroll
+    for (int r = 0; r < FPL; ++r)
+      if (fal[r]) {
+        A.snbr[(size_t)oi * kFaceStage + rt[r]] = tagv[r];
+        A.sarea[(size_t)oi * kFaceStage + rt[r]] = area[r];
+      }
+    if (A.exp_v) {
+      // geometry export for a single-cell query: slots and rings in tag order
+#pragma unroll
+      for (int r = 0; r < VPL; ++r) {
+        const int v = lane + 64 * r;
+        A.exp_v[3 * v] = S.vx[v];
+        A.exp_v[3 * v + 1] = S.vy[v];
+        A.exp_v[3 * v + 2] = S.vz[v];
+        A.exp_meta[2 + v] = (int)((C.VA[r] >> lane) & 1ull);
+      }
+#pragma unroll
+      for (int r = 0; r < FPL; ++r)
+        if (fal[r]) {
+          const int st = S.f_start[lane + 64 * r];
+          int h = st, k = 0;
+          do {
+            A.exp_ring[rt[r] * 64 + k] = S.he_tgt[h];
+            h = S.he_next[h];
+            ++k;
+          } while (h != st && k < 63);
+          A.exp_ring[rt[r] * 64 + k] = -1;
+        }
+      if (lane == 0) {
+        A.exp_meta[0] = L::VC;
+        A.exp_meta[1] = nfaces;
+      }
+    }
+  } else {
+    nfaces = 0;
+  }
+  oflops = wave_sum_u64(oflops);
+  if (lane == 0) {
+    if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) {
+      A.volume[oi] = 0.0;
+      A.fcount[oi] = 0;
+      if ((status & TESS_CELL_CAPACITY) && A.redo) {
+        const uint32_t slot = atomicAdd(A.nredo, 1u);
+        A.redo[slot] = (uint32_t)gpos;
+      }
+    }
+    A.status[oi] = status;
+    if (A.stats) {
+      tess_cell_stats st;
+      st.streamed = streamed;
+      st.classified = classified;
+      st.cuts = ncuts;
+      st.entries = (uint32_t)e_stop_final;
+      st.flops = flops + oflops + 1;
+      st.r2 = r2;
+      st.max_vertices = (uint32_t)maxv;
+      st.status = status;
+      A.stats[oi] = st;
+    }
+  }
+  (void)FC;
+}
+
+template __global__ void k_cells<1, 2, 1>(CellArgs);
+template __global__ void k_cells<2, 3, 2>(CellArgs);
+template __global__ void k_cells<4, 6, 2>(CellArgs);
+
+// f64 sqrt / reciprocal / division as compiled for the cell kernel.
+__global__ void k_diag_arith(const double* a, const double* b, int n, double* s, double* r, double* q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  s[i] = sqrt(a[i]);
+  r[i] = 1.0 / a[i];
+  q[i] = a[i] / b[i];
+}
+
+}  // namespace tess
+
+namespace tess {
+
+hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
+  if (a.nwork <= 0) return hipSuccess;
+  int nb = a.nwork;
+  if (a.xcd_remap) nb = (nb + 7) & ~7;
+  switch (vpl) {
+    case 1:
+      hipLaunchKernelGGL((k_cells<1, 2, 1>), dim3(nb), dim3(64), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_cells<2, 3, 2>), dim3(nb), dim3(64), 0, s, a);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_cells<4, 6, 2>), dim3(nb), dim3(64), 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_arith(const double* a, const double* b, int n, double* sq, double* rc,
+                             double* dv, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_arith, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n, sq, rc, dv);
+  return hipGetLastError();
+}
+
+}  // namespace tess

@@ -1,0 +1,66 @@
+// tess_internal.h -- structures shared by the host runtime and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tess.h"
+
+namespace tess {
+
+// Uniform grid over the points' bounding box (celery.rs:81-188).
+struct GridDev {
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  double sx, sy, sz;  // cell sizes
+  double ix, iy, iz;  // inverse cell sizes
+  int32_t cpd;
+  int32_t pad;
+};
+
+// Shell-table entry offset packed as three 10-bit signed fields.
+__host__ __device__ inline int32_t pack_off(int ox, int oy, int oz) {
+  return ((ox + 512) << 20) | ((oy + 512) << 10) | (oz + 512);
+}
+__host__ __device__ inline void unpack_off(int32_t p, int& ox, int& oy, int& oz) {
+  ox = ((p >> 20) & 1023) - 512;
+  oy = ((p >> 10) & 1023) - 512;
+  oz = (p & 1023) - 512;
+}
+
+// Faces staged per cell before CSR compaction.
+constexpr int kFaceStage = 64;
+
+struct CellArgs {
+  int n;                   // points in the set
+  const double* spx;       // positions in grid order
+  const double* spy;
+  const double* spz;
+  const uint32_t* sorted;  // grid order -> original index
+  const uint32_t* delim;   // CSR delimiters, cpd^3 + 1
+  const GridDev* grid;
+  const double* tkey;      // shell table (canonical order)
+  const int32_t* toff;
+  int tlen;
+  double kcut;             // every entry past the table has key >= kcut
+  double lo[3], hi[3];     // container box
+  const uint32_t* work;    // grid positions to compute (nullptr: 0..nwork-1)
+  int nwork;
+  int xcd_remap;           // 1: give each XCD a contiguous range of cells
+  const int64_t* groups;   // nullable
+  int64_t target_group;    // < 0: none
+  double radius_limit;     // NaN: none (expand_all_in_radius cut-off otherwise)
+  // outputs, indexed by original point index
+  double* volume;
+  uint32_t* fcount;
+  int32_t* snbr;           // [n][kFaceStage]
+  double* sarea;           // [n][kFaceStage]
+  uint32_t* status;
+  tess_cell_stats* stats;  // nullable
+  uint32_t* redo;          // grid positions of cells needing a larger variant
+  uint32_t* nredo;
+  // single-cell geometry export (nullable): vertices and face rings
+  double* exp_v;           // [256][3]
+  int32_t* exp_ring;       // [256 faces][64] vertex ids, -1 terminated
+  int32_t* exp_meta;       // nv, nf
+};
+
+}  // namespace tess

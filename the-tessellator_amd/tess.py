@@ -1,0 +1,336 @@
+"""ctypes binding of libtess (include/tess.h) and of the synthetic workloads.
+
+The classes mirror the reference's interface.rs surface:
+``Diagram`` (interface.rs:24-232), ``Cell`` (interface.rs:236-390) and the
+per-face accessors of ``VoronoiFace`` (interface.rs:393-417).  Every call goes
+through the C ABI of ``libtess.so``; a missing library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtess.so")
+DATA_PATH = os.path.join(HERE, "libtessdata.so")
+
+TESS_OK = 0
+STATUS = {0: "TESS_OK", 1: "TESS_E_INVALID", 2: "TESS_E_NOMEM", 3: "TESS_E_DEVICE",
+          4: "TESS_E_CAPACITY", 5: "TESS_E_DEGENERATE", 6: "TESS_E_STATE", 7: "TESS_E_NOTFOUND"}
+MODE_PLAIN, MODE_LEARNED = 0, 1
+CELL_CAPACITY, CELL_TABLE, CELL_DEGENERATE, CELL_FALLBACK = 1, 2, 4, 8
+
+EXPORTS = [
+    "tess_opts_default", "tess_ctx_create", "tess_ctx_destroy", "tess_last_error", "tess_ctx_stream",
+    "tess_ctx_synchronize", "tess_diagram_create", "tess_diagram_create_device", "tess_diagram_destroy",
+    "tess_diagram_size", "tess_diagram_compute", "tess_diagram_total_faces", "tess_diagram_copy_cells",
+    "tess_diagram_device_cells", "tess_diagram_copy_stats", "tess_diagram_timings", "tess_diagram_grid",
+    "tess_diagram_copy_grid", "tess_cell_at_index", "tess_cell_destroy", "tess_cell_original_index",
+    "tess_cell_volume", "tess_cell_neighbors", "tess_cell_face_count", "tess_cell_face",
+    "tess_cell_face_vertices", "tess_cell_vertices", "tess_diag_arith",
+]
+
+
+class TessError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Opts(C.Structure):
+    _fields_ = [("cell_density", C.c_double), ("mode", C.c_int32), ("vertex_capacity", C.c_int32),
+                ("collect_stats", C.c_int32), ("reserved", C.c_int32)]
+
+
+class CellStats(C.Structure):
+    _fields_ = [("streamed", C.c_uint32), ("classified", C.c_uint32), ("cuts", C.c_uint32),
+                ("entries", C.c_uint32), ("flops", C.c_uint64), ("r2", C.c_double),
+                ("max_vertices", C.c_uint32), ("status", C.c_uint32)]
+
+
+STATS_DTYPE = np.dtype([("streamed", "<u4"), ("classified", "<u4"), ("cuts", "<u4"), ("entries", "<u4"),
+                        ("flops", "<u8"), ("r2", "<f8"), ("max_vertices", "<u4"), ("status", "<u4")])
+assert STATS_DTYPE.itemsize == C.sizeof(CellStats)
+
+
+class Timings(C.Structure):
+    _fields_ = [("bin_ms", C.c_double), ("cells_ms", C.c_double), ("compact_ms", C.c_double),
+                ("total_ms", C.c_double), ("table_ms", C.c_double), ("flops", C.c_uint64),
+                ("fallback_cells", C.c_uint64), ("capacity_reruns", C.c_uint64)]
+
+
+_lib = None
+_data = None
+
+
+def load_library(path: str | None = None) -> C.CDLL:
+    """Load libtess.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(f"{p} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(p)
+    vp, sz, dp = C.c_void_p, C.c_size_t, C.c_void_p
+    sig = {
+        "tess_opts_default": (None, [vp]),
+        "tess_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "tess_ctx_destroy": (None, [vp]),
+        "tess_last_error": (C.c_char_p, [vp]),
+        "tess_ctx_stream": (vp, [vp]),
+        "tess_ctx_synchronize": (C.c_int, [vp]),
+        "tess_diagram_create": (C.c_int, [vp, dp, sz, dp, dp, dp, vp, C.POINTER(vp)]),
+        "tess_diagram_create_device": (C.c_int, [vp, vp, sz, dp, dp, vp, vp, C.POINTER(vp)]),
+        "tess_diagram_destroy": (None, [vp]),
+        "tess_diagram_size": (sz, [vp]),
+        "tess_diagram_compute": (C.c_int, [vp]),
+        "tess_diagram_total_faces": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+        "tess_diagram_copy_cells": (C.c_int, [vp, dp, dp, dp, dp, dp]),
+        "tess_diagram_device_cells": (C.c_int, [vp, vp]),
+        "tess_diagram_copy_stats": (C.c_int, [vp, dp]),
+        "tess_diagram_timings": (C.c_int, [vp, C.POINTER(Timings)]),
+        "tess_diagram_grid": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int64)]),
+        "tess_diagram_copy_grid": (C.c_int, [vp, dp, dp, dp]),
+        "tess_cell_at_index": (C.c_int, [vp, sz, C.c_double, C.c_int64, C.POINTER(vp)]),
+        "tess_cell_destroy": (None, [vp]),
+        "tess_cell_original_index": (C.c_int64, [vp]),
+        "tess_cell_volume": (C.c_double, [vp]),
+        "tess_cell_neighbors": (sz, [vp, dp, sz]),
+        "tess_cell_face_count": (sz, [vp]),
+        "tess_cell_face": (C.c_int, [vp, sz, dp, dp]),
+        "tess_cell_face_vertices": (sz, [vp, sz, dp, sz]),
+        "tess_cell_vertices": (sz, [vp, dp, sz]),
+        "tess_diag_arith": (C.c_int, [vp, dp, dp, sz, dp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """One HIP device + stream (tess_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.tess_ctx_create(device, C.byref(h))
+        if st != TESS_OK:
+            raise TessError(st, f"cannot open HIP device {device}")
+        self.h = h
+
+    def check(self, st):
+        if st != TESS_OK:
+            raise TessError(st, self.lib.tess_last_error(self.h).decode())
+
+    def synchronize(self):
+        self.check(self.lib.tess_ctx_synchronize(self.h))
+
+    def diag_arith(self, a, b):
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        s, r, q = (np.empty_like(a) for _ in range(3))
+        self.check(self.lib.tess_diag_arith(self.h, _ptr(a), _ptr(b), a.size, _ptr(s), _ptr(r), _ptr(q)))
+        return s, r, q
+
+    def close(self):
+        if self.h:
+            self.lib.tess_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Cell:
+    """A computed cell (interface.rs:236-390); faces are VoronoiFace views."""
+
+    def __init__(self, lib, h):
+        self.lib, self.h = lib, h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.tess_cell_destroy(self.h)
+            self.h = None
+
+    def original_index(self):
+        return int(self.lib.tess_cell_original_index(self.h))
+
+    def compute_volume(self):
+        return float(self.lib.tess_cell_volume(self.h))
+
+    def compute_neighbors(self):
+        n = self.lib.tess_cell_neighbors(self.h, None, 0)
+        out = np.empty(n, np.int64)
+        self.lib.tess_cell_neighbors(self.h, _ptr(out), n)
+        return out
+
+    def face_count(self):
+        return int(self.lib.tess_cell_face_count(self.h))
+
+    def face(self, f):
+        area, nb = C.c_double(), C.c_int64()
+        st = self.lib.tess_cell_face(self.h, f, C.byref(area), C.byref(nb))
+        if st != TESS_OK:
+            raise TessError(st, "bad face index")
+        return area.value, nb.value
+
+    def face_vertices(self, f):
+        k = self.lib.tess_cell_face_vertices(self.h, f, None, 0)
+        out = np.empty((k, 3))
+        self.lib.tess_cell_face_vertices(self.h, f, _ptr(out), k)
+        return out
+
+    def compute_vertices(self):
+        k = self.lib.tess_cell_vertices(self.h, None, 0)
+        out = np.empty((k, 3))
+        self.lib.tess_cell_vertices(self.h, _ptr(out), k)
+        return out
+
+
+class Diagram:
+    """Points + container box (tess_diagram, interface.rs:24-84)."""
+
+    def __init__(self, ctx: Context, xyz, lo, hi, groups=None, density=1.25, mode=MODE_PLAIN,
+                 vertex_capacity=0, collect_stats=False):
+        self.ctx, self.lib = ctx, ctx.lib
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        self.n = xyz.shape[0]
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        g = None if groups is None else np.ascontiguousarray(groups, np.int64)
+        o = Opts()
+        self.lib.tess_opts_default(C.byref(o))
+        o.cell_density, o.mode, o.vertex_capacity, o.collect_stats = density, mode, vertex_capacity, int(collect_stats)
+        h = C.c_void_p()
+        ctx.check(self.lib.tess_diagram_create(ctx.h, _ptr(xyz), self.n, _ptr(lo), _ptr(hi), _ptr(g),
+                                               C.byref(o), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.tess_diagram_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute(self):
+        st = self.lib.tess_diagram_compute(self.h)
+        if st != TESS_OK:
+            raise TessError(st, self.lib.tess_last_error(self.ctx.h).decode())
+
+    def compute_status(self):
+        """Run compute and return the status code instead of raising."""
+        return self.lib.tess_diagram_compute(self.h)
+
+    def cells(self):
+        tf = C.c_uint64()
+        self.ctx.check(self.lib.tess_diagram_total_faces(self.h, C.byref(tf)))
+        n, f = self.n, tf.value
+        vol, fc = np.empty(n), np.empty(n, np.uint32)
+        fo, nb, ar = np.empty(n + 1, np.uint64), np.empty(f, np.int32), np.empty(f)
+        self.ctx.check(self.lib.tess_diagram_copy_cells(self.h, _ptr(vol), _ptr(fc), _ptr(fo), _ptr(nb), _ptr(ar)))
+        return dict(volume=vol, face_count=fc, face_offset=fo, neighbor=nb, area=ar)
+
+    def stats(self):
+        out = np.empty(self.n, STATS_DTYPE)
+        self.ctx.check(self.lib.tess_diagram_copy_stats(self.h, _ptr(out)))
+        return out
+
+    def timings(self):
+        t = Timings()
+        self.ctx.check(self.lib.tess_diagram_timings(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timings._fields_}
+
+    def grid(self):
+        b, s, cpd = np.empty(6), np.empty(3), C.c_int64()
+        self.ctx.check(self.lib.tess_diagram_grid(self.h, _ptr(b), _ptr(s), C.byref(cpd)))
+        return b, s, cpd.value
+
+    def copy_grid(self):
+        _, _, cpd = self.grid()
+        cell = np.empty(self.n, np.uint32)
+        srt = np.empty(self.n, np.uint32)
+        dl = np.empty(cpd ** 3 + 1, np.uint64)
+        self.ctx.check(self.lib.tess_diagram_copy_grid(self.h, _ptr(cell), _ptr(srt), _ptr(dl)))
+        return cell, srt, dl
+
+    def get_cell_at_index(self, index, search_radius=None, target_group=None):
+        """interface.rs:186-208 + Cell::compute_voronoi_cell."""
+        h = C.c_void_p()
+        r = math.nan if search_radius is None else float(search_radius)
+        g = -1 if target_group is None else int(target_group)
+        self.ctx.check(self.lib.tess_cell_at_index(self.h, index, r, g, C.byref(h)))
+        return Cell(self.lib, h)
+
+
+# ------------------------------------------------------------ workloads
+def _data_lib():
+    global _data
+    if _data is None:
+        if not os.path.exists(DATA_PATH):
+            raise OSError(f"{DATA_PATH} is missing: build it first")
+        d = C.CDLL(DATA_PATH)
+        d.tg_uniform.argtypes = [C.c_uint64, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
+        d.tg_clustered.argtypes = [C.c_uint64, C.c_size_t, C.c_int, C.c_double, C.c_double, C.c_void_p]
+        d.tg_batch_shape.argtypes = [C.c_uint64, C.c_double, C.c_double, C.POINTER(C.c_uint64), C.c_void_p]
+        d.tg_batch_points.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_void_p]
+        d.tg_fcc_count.argtypes = [C.c_int]
+        d.tg_fcc_count.restype = C.c_size_t
+        d.tg_fcc.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_void_p]
+        _data = d
+    return _data
+
+
+def uniform(n, seed, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """n points i.i.d. uniform in [lo, hi) (xoshiro256** via SplitMix64)."""
+    d = _data_lib()
+    out = np.empty((n, 3))
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    d.tg_uniform(seed, n, _ptr(lo), _ptr(hi), _ptr(out))
+    return out
+
+
+def clustered(n, seed, nclusters=64, sigma=0.02, frac=0.8):
+    """80% from 64 Gaussian clusters (sigma 0.02, resampled inside the unit cube), 20% uniform."""
+    d = _data_lib()
+    out = np.empty((n, 3))
+    d.tg_clustered(seed, n, nclusters, sigma, frac, _ptr(out))
+    return out
+
+
+def batch_set(seed, nmin=1000.0, nmax=200000.0):
+    """One uneven batch set: (points, box side)."""
+    d = _data_lib()
+    n = C.c_uint64()
+    side = np.empty(3)
+    d.tg_batch_shape(seed, nmin, nmax, C.byref(n), _ptr(side))
+    out = np.empty((n.value, 3))
+    d.tg_batch_points(seed, nmin, nmax, _ptr(out))
+    return out, side
+
+
+def fcc(seed, ncell=63, a=1.0, sigma=1e-3):
+    """ncell^3 FCC cells (cubic constant a), 4 sites each, Gaussian jitter sigma."""
+    d = _data_lib()
+    out = np.empty((d.tg_fcc_count(ncell), 3))
+    d.tg_fcc(seed, ncell, a, sigma, _ptr(out))
+    return out
