@@ -227,6 +227,7 @@ struct tess_diagram {
   DevBuf<int32_t> snbr, nbr;
   DevBuf<uint64_t> foff, fsums;
   DevBuf<tess_cell_stats> stats;
+  DevBuf<Summary> summary;
   DevBuf<uint32_t> work1;
   DevBuf<double> exp_v;
   DevBuf<int32_t> exp_ring, exp_meta;
@@ -240,7 +241,7 @@ namespace {
 tess_status alloc_all(tess_diagram* d) {
   tess_ctx* c = d->ctx;
   const size_t n = d->n;
-  const int nparts = (int)std::min<size_t>(1024, (n + 255) / 256);
+  const int nparts = (int)std::min<size_t>(256, (n + 255) / 256);
   const size_t nb = (size_t)d->nbins;
   bool ok = d->x.alloc(n) == hipSuccess && d->y.alloc(n) == hipSuccess && d->z.alloc(n) == hipSuccess &&
             d->grid.alloc(1) == hipSuccess && d->part.alloc(6 * (size_t)nparts) == hipSuccess &&
@@ -253,7 +254,8 @@ tess_status alloc_all(tess_diagram* d) {
             d->fcount.alloc(n) == hipSuccess && d->status.alloc(n) == hipSuccess &&
             d->redo.alloc(n + 1) == hipSuccess && d->snbr.alloc(n * kFaceStage) == hipSuccess &&
             d->sarea.alloc(n * kFaceStage) == hipSuccess && d->foff.alloc(n + 1) == hipSuccess &&
-            d->fsums.alloc((n + 1023) / 1024 + 1) == hipSuccess && d->work1.alloc(1) == hipSuccess;
+            d->fsums.alloc((n + 1023) / 1024 + 1) == hipSuccess && d->work1.alloc(1) == hipSuccess &&
+            d->summary.alloc(1) == hipSuccess;
   if (ok && d->opts.collect_stats) ok = d->stats.alloc(n) == hipSuccess;
   if (!ok) return c->fail(TESS_E_NOMEM, "device allocation failed");
   return TESS_OK;
@@ -263,7 +265,7 @@ tess_status bin(tess_diagram* d) {
   tess_ctx* c = d->ctx;
   hipStream_t s = c->stream;
   const int n = (int)d->n;
-  const int nparts = (int)std::min<size_t>(1024, (d->n + 255) / 256);
+  const int nparts = (int)std::min<size_t>(256, (d->n + 255) / 256);
   tess_status st;
   if ((st = c->hip(launch_bounds(d->x.p, d->y.p, d->z.p, n, (int)d->cpd, d->part.p, nparts, d->grid.p, s),
                    "bounds")) != TESS_OK)
@@ -292,14 +294,16 @@ tess_status bin(tess_diagram* d) {
   return TESS_OK;
 }
 
+// Vertex capacity of the first pass (64 by default: highest occupancy);
+// cells that outgrow it are rerun with 256.
 int pick_vpl(const tess_diagram* d) {
   switch (d->opts.vertex_capacity) {
-    case 64:
-      return 1;
+    case 128:
+      return 2;
     case 256:
       return 4;
     default:
-      return 2;
+      return 1;
   }
 }
 
@@ -330,6 +334,7 @@ CellArgs base_args(tess_diagram* d, const Table* t) {
   a.sarea = d->sarea.p;
   a.status = d->status.p;
   a.stats = d->opts.collect_stats ? d->stats.p : nullptr;
+  a.summary = d->summary.p;
   return a;
 }
 
@@ -533,6 +538,10 @@ tess_status tess_diagram_compute(tess_diagram* d) {
   }
   const int n = (int)d->n;
   const int vpl = pick_vpl(d);
+  if ((st = c->hip(hipMemsetAsync(d->summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK) {
+    cleanup();
+    return st;
+  }
   if ((st = run_cells(d, t, vpl, nullptr, n)) != TESS_OK) {
     cleanup();
     return st;
@@ -596,25 +605,16 @@ tess_status tess_diagram_compute(tess_diagram* d) {
   cleanup();
   d->total_faces = total;
   d->computed = true;
-  // status scan: capacity/degenerate cells are reported
-  std::vector<uint32_t> stv(d->n);
-  if ((st = c->hip(hipMemcpy(stv.data(), d->status.p, 4 * d->n, hipMemcpyDeviceToHost), "status")) != TESS_OK)
+  // launch summary: capacity cells that the rerun fixed were counted once
+  // by the first pass and not again by the rerun, so re-derive them.
+  Summary sm;
+  if ((st = c->hip(hipMemcpy(&sm, d->summary.p, sizeof(Summary), hipMemcpyDeviceToHost), "summary")) != TESS_OK)
     return st;
-  uint64_t bad = 0, cap = 0, tab = 0;
-  for (uint32_t v : stv) {
-    bad += (v & TESS_CELL_DEGENERATE) != 0;
-    cap += (v & TESS_CELL_CAPACITY) != 0;
-    tab += (v & TESS_CELL_TABLE) != 0;
-  }
+  d->tim.flops = sm.flops;
+  const uint64_t bad = sm.degenerate, tab = sm.table;
+  const bool reran = nredo && vpl < 4;
+  const uint64_t cap = reran ? sm.capacity - nredo : sm.capacity;
   d->degenerate = bad;
-  if (d->opts.collect_stats) {
-    std::vector<tess_cell_stats> sv(d->n);
-    if (hipMemcpy(sv.data(), d->stats.p, sizeof(tess_cell_stats) * d->n, hipMemcpyDeviceToHost) == hipSuccess) {
-      uint64_t f = 0;
-      for (auto& x : sv) f += x.flops;
-      d->tim.flops = f;
-    }
-  }
   if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
   if (tab) return c->fail(TESS_E_CAPACITY, std::to_string(tab) + " cells exhausted the shell table");
   if (bad) return c->fail(TESS_E_DEGENERATE, std::to_string(bad) + " cells failed topology checks");

@@ -75,17 +75,28 @@ __global__ __launch_bounds__(256) void k_bounds_partial(const double* __restrict
   if (t < 6) part[blockIdx.x * 6 + t] = sm[t][0];
 }
 
-// Reduce the partials; cell sizes as celery.rs:164-170: (max-min)/cpd and cpd/(max-min).
-__global__ void k_bounds_final(const double* __restrict__ part, int nparts, int cpd,
-                               GridDev* __restrict__ g) {
-  if (threadIdx.x != 0) return;
+// Reduce the partials (one wavefront); cell sizes as celery.rs:164-170:
+// (max-min)/cpd and cpd/(max-min).
+__global__ __launch_bounds__(64) void k_bounds_final(const double* __restrict__ part, int nparts, int cpd,
+                                                     GridDev* __restrict__ g) {
+  const int t = threadIdx.x;
   double lo[3] = {part[0], part[1], part[2]}, hi[3] = {part[3], part[4], part[5]};
-  for (int b = 1; b < nparts; ++b)
+  for (int b = t; b < nparts; b += 64)
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
       const double l = part[6 * b + a], h = part[6 * b + 3 + a];
       lo[a] = l < lo[a] ? l : lo[a];
       hi[a] = h > hi[a] ? h : hi[a];
     }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double l = __shfl_xor(lo[a], o, 64), h = __shfl_xor(hi[a], o, 64);
+      lo[a] = l < lo[a] ? l : lo[a];
+      hi[a] = h > hi[a] ? h : hi[a];
+    }
+  if (t != 0) return;
   const double c = (double)cpd;
   g->xmin = lo[0];
   g->xmax = hi[0];

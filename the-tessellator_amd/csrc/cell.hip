@@ -38,12 +38,27 @@ constexpr double kTol = 1e-12;  // polyhedron.rs:221-223
 enum : uint8_t { C_OUT = 0, C_INC = 1, C_IN = 2 };
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// One wavefront per workgroup: the wave's LDS operations execute in issue
+// order, so a phase boundary only has to stop the compiler from moving LDS
+// accesses across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 __device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
 __device__ __forceinline__ int ctz64(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
 __device__ __forceinline__ int mbcnt(uint64_t m, int lane) {
   return __popcll(m & ((1ull << lane) - 1ull));
 }
 __device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// A value every lane holds (e.g. an LDS broadcast read) as a scalar.
+__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni_d(double v) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
+  int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 __device__ __forceinline__ double rl_d(double v, int l) {
   unsigned long long u = (unsigned long long)__double_as_longlong(v);
   int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
@@ -150,7 +165,17 @@ template <int VPL, int PPL, int FPL>
 struct Lds {
   static constexpr int VC = 64 * VPL, PC = 64 * PPL, HC = 2 * PC, FC = 64 * FPL;
   double vx[VC], vy[VC], vz[VC], vdot[VC];
-  double fcontrib[FC];
+  // kept candidates of the current chunk (in stream order); the output
+  // phase reuses the space for per-face volume terms
+  union {
+    struct {
+      double n[64][4];  // nx, ny, nz, offset
+      double d2[64];
+      int32_t tag[64];
+      int32_t ent[64];
+    } cand;
+    double fcontrib[FC > 64 ? FC : 64];
+  } u;
   int32_t f_tag[FC];
   uint32_t bstart[64];
   uint32_t bpref[64];
@@ -158,6 +183,8 @@ struct Lds {
   uint16_t f_start[FC];
   uint16_t f_seq[FC];
   uint16_t exit_of[VC];
+  uint16_t fentry[FC];  // fast clip: the Outside-run entry half-edge of each face
+  uint16_t fexit[FC];   // ... and its exit half-edge
   uint16_t run_a[64];
   uint8_t he_tgt[HC];
   uint8_t he_face[HC];
@@ -166,7 +193,7 @@ struct Lds {
   uint8_t fmark[FC];
 };
 
-enum { CUT_OK = 0, CUT_CAPACITY = 1, CUT_DEGENERATE = 2 };
+enum { CUT_OK = 0, CUT_CAPACITY = 1, CUT_DEGENERATE = 2, CUT_GENERAL = 3 };
 
 template <int VPL, int PPL, int FPL>
 struct CellState {
@@ -177,10 +204,191 @@ struct CellState {
   double rx[VPL], ry[VPL], rz[VPL];  // this lane's vertex slots (register copy)
 };
 
+// Rank of pair p among the set bits of m (bits below p).
+template <int W>
+__device__ __forceinline__ int rank_multi(const uint64_t (&m)[W], int p) {
+  int c = 0;
+#pragma unroll
+  for (int r = 0; r < W; ++r) {
+    if (r < (p >> 6))
+      c += __popcll(m[r]);
+    else if (r == (p >> 6))
+      c += __popcll(m[r] & ((1ull << (p & 63)) - 1ull));
+  }
+  return c;
+}
+
+// Clip for the common configuration: no Incident vertex is adjacent to an
+// Outside one and every face crossed by the plane has one Outside run.  Then
+// every Inside-Outside edge pair is both the entry of a run (on the face
+// where it goes Inside -> Outside) and the exit of another (on the other
+// face), so the pairs index everything: pair k of the crossing set owns new
+// vertex k (its intersection point) and bridge pair k (the bridge on its
+// entry face and that bridge's flip on the new face).  Returns CUT_GENERAL
+// without touching the polyhedron when the configuration is not the common
+// one.
+template <int VPL, int PPL, int FPL>
+__device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C,
+                                                   const uint64_t (&OM)[VPL], double off, int32_t tag, int lane,
+                                                   int* newv) {
+  // per pair: targets and faces packed in one word, classes in two bytes
+  uint32_t pk[PPL];
+  uint32_t cls[PPL];
+  uint64_t CM[PPL], PD[PPL];
+  bool slow = false;
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    const int p = lane + 64 * r;
+    const bool al = (C.PA[r] >> lane) & 1ull;
+    uint32_t t0 = 0, t1 = 0, f0 = 0, f1 = 0, c0 = C_IN, c1 = C_IN;
+    if (al) {
+      t0 = S.he_tgt[2 * p];
+      t1 = S.he_tgt[2 * p + 1];
+      f0 = S.he_face[2 * p];
+      f1 = S.he_face[2 * p + 1];
+      c0 = S.vcls[t0];
+      c1 = S.vcls[t1];
+    }
+    pk[r] = t0 | (t1 << 8) | (f0 << 16) | (f1 << 24);
+    cls[r] = c0 | (c1 << 8);
+    const bool o0 = c0 == C_OUT, o1 = c1 == C_OUT;
+    const bool cross = al && ((o0 && c1 == C_IN) || (o1 && c0 == C_IN));
+    slow = slow || (al && ((o0 && c1 == C_INC) || (o1 && c0 == C_INC)));
+    CM[r] = ballot(cross);
+    PD[r] = ballot(al && (o0 || o1) && !cross);
+  }
+  if (ballot(slow)) return CUT_GENERAL;
+  const int ncross = popc_multi(CM);
+  if (ncross < 3) return CUT_GENERAL;
+  // new vertices go to slots that hold no vertex now, so they can be written
+  // before the clip commits; bridges may reuse the pairs that die
+  uint64_t vfree[VPL], pfree[PPL];
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) vfree[r] = ~C.VA[r];
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) pfree[r] = ~(C.PA[r] & ~PD[r]);
+  if (ncross > popc_multi(vfree) || ncross > popc_multi(pfree)) return CUT_CAPACITY;
+  int nf = -1;
+#pragma unroll
+  for (int r = FPL - 1; r >= 0; --r)
+    if (~C.FA[r]) nf = 64 * r + ctz64(~C.FA[r]);
+  if (nf < 0) return CUT_CAPACITY;
+
+  // crossing pairs: intersection(outside, inside) into new slots, face maps
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    if ((CM[r] >> lane) & 1ull) {
+      const int p = lane + 64 * r;
+      const bool first_out = (cls[r] & 0xff) == C_OUT;  // half-edge 2p points at the Outside end
+      const int t0 = pk[r] & 0xff, t1 = (pk[r] >> 8) & 0xff;
+      const int f0 = (pk[r] >> 16) & 0xff, f1 = pk[r] >> 24;
+      const int o = first_out ? t0 : t1, i = first_out ? t1 : t0;
+      const double dO = S.vdot[o], dI = S.vdot[i];
+      const double t = (off - dO) / (dI - dO);
+      const double ox = S.vx[o], oy = S.vy[o], oz = S.vz[o];
+      const int sv = select_multi(vfree, rank_multi(CM, p));
+      S.vx[sv] = ox + (S.vx[i] - ox) * t;
+      S.vy[sv] = oy + (S.vy[i] - oy) * t;
+      S.vz[sv] = oz + (S.vz[i] - oz) * t;
+      const int hin = first_out ? 2 * p : 2 * p + 1;
+      S.fentry[first_out ? f0 : f1] = (uint16_t)hin;
+      S.fexit[first_out ? f1 : f0] = (uint16_t)(hin ^ 1);
+    }
+  }
+  wave_sync();
+  // one run per face: every writer must read back its own half-edge
+  bool clash = false;
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    if ((CM[r] >> lane) & 1ull) {
+      const int p = lane + 64 * r;
+      const bool first_out = (cls[r] & 0xff) == C_OUT;
+      const int f0 = (pk[r] >> 16) & 0xff, f1 = pk[r] >> 24;
+      const int hin = first_out ? 2 * p : 2 * p + 1;
+      clash = clash || S.fentry[first_out ? f0 : f1] != hin || S.fexit[first_out ? f1 : f0] != (hin ^ 1);
+    }
+  }
+  if (ballot(clash)) return CUT_GENERAL;
+  // commit: bridge k on the entry face of crossing pair k, its flip on the new face
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    if ((CM[r] >> lane) & 1ull) {
+      const int p = lane + 64 * r;
+      const bool first_out = (cls[r] & 0xff) == C_OUT;
+      const int f0 = (pk[r] >> 16) & 0xff, f1 = pk[r] >> 24;
+      const int fin = first_out ? f0 : f1, fout = first_out ? f1 : f0;
+      const int hin = first_out ? 2 * p : 2 * p + 1;
+      const int b = S.fexit[fin];    // exit of the run on the entry face
+      const int a2 = S.fentry[fout]; // entry of the run on the exit face
+      const int k = rank_multi(CM, p);
+      const int E = select_multi(vfree, k);
+      const int Xe = select_multi(vfree, rank_multi(CM, b >> 1));
+      const int br = 2 * select_multi(pfree, k);
+      const int nxt = 2 * select_multi(pfree, rank_multi(CM, a2 >> 1)) + 1;
+      S.he_tgt[hin] = (uint8_t)E;
+      S.he_next[hin] = (uint16_t)br;
+      S.he_tgt[br] = (uint8_t)Xe;
+      S.he_face[br] = (uint8_t)fin;
+      S.he_next[br] = (uint16_t)b;
+      S.he_tgt[br + 1] = (uint8_t)E;
+      S.he_face[br + 1] = (uint8_t)nf;
+      S.he_next[br + 1] = (uint16_t)nxt;
+      S.f_start[fin] = (uint16_t)br;
+    }
+  }
+  uint64_t vnew[VPL], pnew[PPL];
+  lowest_bits(vfree, ncross, vnew);
+  lowest_bits(pfree, ncross, pnew);
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) C.VA[r] = (C.VA[r] & ~OM[r]) | vnew[r];
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) C.PA[r] = (C.PA[r] & ~PD[r]) | pnew[r];
+  if (lane == 0) {
+    S.f_tag[nf] = tag;
+    S.f_seq[nf] = (uint16_t)C.seq;
+    S.f_start[nf] = (uint16_t)(2 * select_multi(pfree, 0) + 1);
+  }
+  C.seq++;
+  // faces whose every vertex was Outside (only possible when pairs died)
+  int anyd = 0;
+#pragma unroll
+  for (int r = 0; r < PPL; ++r) anyd |= (PD[r] != 0);
+  if (anyd) {
+#pragma unroll
+    for (int r = 0; r < FPL; ++r) S.fmark[lane + 64 * r] = 0;
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) {
+      const int p = lane + 64 * r;
+      if ((C.PA[r] >> lane) & 1ull) {
+        S.fmark[S.he_face[2 * p]] = 1;
+        S.fmark[S.he_face[2 * p + 1]] = 1;
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < FPL; ++r) C.FA[r] = ballot(S.fmark[lane + 64 * r] != 0);
+  } else {
+    C.FA[nf >> 6] |= 1ull << (nf & 63);
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) {
+    const int v = lane + 64 * r;
+    if ((vnew[r] >> lane) & 1ull) {
+      C.rx[r] = S.vx[v];
+      C.ry[r] = S.vy[v];
+      C.rz[r] = S.vz[v];
+    }
+  }
+  *newv = ncross;
+  return CUT_OK;
+}
+
 // Apply one bisector that puts at least one vertex Outside.  OM: Outside
 // vertices; S.vcls / S.vdot hold every live vertex's class and n.v.
 template <int VPL, int PPL, int FPL>
-__device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const uint64_t (&OM)[VPL],
+__device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const uint64_t (&OM)[VPL],
                     double off, int32_t tag, int lane, int* newv) {
   constexpr int HC = Lds<VPL, PPL, FPL>::HC;
 #pragma unroll
@@ -216,7 +424,7 @@ __device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const ui
   }
   if (nruns > 64) return CUT_CAPACITY;
   if (nruns < 1) return CUT_DEGENERATE;
-  __syncthreads();
+  wave_sync();
 
   // 2. one lane per run: walk to the exit, classify the run's ends.
   const bool isrun = lane < nruns;
@@ -297,14 +505,14 @@ __device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const ui
     X1 = oy + (S.vy[Q] - oy) * t;
     X2 = oz + (S.vz[Q] - oz) * t;
   }
-  __syncthreads();
+  wave_sync();
   if (needX) {
     S.vx[vslot] = X0;
     S.vy[vslot] = X1;
     S.vz[vslot] = X2;
     S.newv_of[b] = (uint8_t)vslot;
   }
-  __syncthreads();
+  wave_sync();
   const int Xid = needX ? vslot : Q;
   const int Eid = isrun ? ((cP == C_IN) ? (int)S.newv_of[a ^ 1] : P) : 0;
 
@@ -331,7 +539,7 @@ __device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const ui
     S.exit_of[Q] = (uint16_t)nb;
     nfh = nb;
   }
-  __syncthreads();
+  wave_sync();
   bool lbad = false;
   if (nfh >= 0) {
     const int nx = S.exit_of[Eid];
@@ -361,7 +569,7 @@ __device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const ui
     S.f_start[nf] = (uint16_t)nfstart;
   }
   C.seq++;
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
     const int p = lane + 64 * r;
@@ -370,7 +578,7 @@ __device__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const ui
       S.fmark[S.he_face[2 * p + 1]] = 1;
     }
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int r = 0; r < FPL; ++r) C.FA[r] = ballot(S.fmark[lane + 64 * r] != 0);
   // refresh this lane's vertex registers
@@ -396,7 +604,7 @@ __device__ double cell_r2(const CellState<VPL, PPL, FPL>& C, int lane) {
       double d = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
       m = d > m ? d : m;
     }
-  return wave_max(m);
+  return uni_d(wave_max(m));  // every lane holds the maximum; keep it scalar
 }
 
 }  // namespace
@@ -465,7 +673,7 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
     C.FA[0] = 0x3full;
     C.seq = 6;
   }
-  __syncthreads();
+  wave_sync();
   double r2 = cell_r2(C, lane);
 
   uint32_t streamed = 0, classified = 0, ncuts = 0, status = 0;
@@ -511,7 +719,7 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
     const int total = rl_i(incl, 63);
     S.bpref[lane] = (uint32_t)(incl - c);
     S.bstart[lane] = bst;
-    __syncthreads();
+    wave_sync();
 
     for (int base = 0; base < total && !done; base += 64) {
       const int k = base + lane;
@@ -548,15 +756,28 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
         nz = pz * inv;
         off = (nx * (px * 0.5) + ny * (py * 0.5)) + nz * (pz * 0.5);
       }
-      uint64_t km = ballot(keep);
-      while (km) {
-        const int cl = ctz64(km);
-        km &= km - 1;
-        const int ec = ebase + rl_i(ek, cl);
+      // stage the kept candidates in stream order; the sequential pass reads
+      // them back as wave-uniform values
+      const uint64_t km = ballot(keep);
+      if (keep) {
+        const int q = mbcnt(km, lane);
+        S.u.cand.n[q][0] = nx;
+        S.u.cand.n[q][1] = ny;
+        S.u.cand.n[q][2] = nz;
+        S.u.cand.n[q][3] = off;
+        S.u.cand.d2[q] = d2;
+        S.u.cand.tag[q] = (int32_t)jidx;
+        S.u.cand.ent[q] = eg;
+      }
+      const int nkeep = popc(km);
+      wave_sync();
+      for (int q = 0; q < nkeep; ++q) {
+        const int ec = uni_i(S.u.cand.ent[q]);
         if (ec >= e_stop) break;
-        const double cd2 = rl_d(d2, cl);
+        const double cd2 = uni_d(S.u.cand.d2[q]);
         if (cd2 > stop_thr(r2)) continue;
-        const double cnx = rl_d(nx, cl), cny = rl_d(ny, cl), cnz = rl_d(nz, cl), coff = rl_d(off, cl);
+        const double cnx = uni_d(S.u.cand.n[q][0]), cny = uni_d(S.u.cand.n[q][1]);
+        const double cnz = uni_d(S.u.cand.n[q][2]), coff = uni_d(S.u.cand.n[q][3]);
         int nv = popc_multi(C.VA);
         classified++;
         flops += 13 + 6ull * (uint64_t)nv;
@@ -571,8 +792,8 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
           double d = 0.0;
           if (al) {
             d = (cnx * C.rx[r] + cny * C.ry[r]) + cnz * C.rz[r];
-            const double s = d - coff;
-            cc = s > kTol ? C_OUT : (s < -kTol ? C_IN : C_INC);
+            const double sd = d - coff;
+            cc = sd > kTol ? C_OUT : (sd < -kTol ? C_IN : C_INC);
           }
           cv[r] = cc;
           dv[r] = d;
@@ -586,9 +807,11 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
             S.vcls[lane + 64 * r] = cv[r];
             S.vdot[lane + 64 * r] = dv[r];
           }
-        __syncthreads();
+        wave_sync();
         int newv = 0;
-        const int rc = clip(S, C, OM, coff, (int32_t)rl_i((int)jidx, cl), lane, &newv);
+        const int32_t ctag = uni_i(S.u.cand.tag[q]);
+        int rc = clip_fast(S, C, OM, coff, ctag, lane, &newv);
+        if (rc == CUT_GENERAL) rc = clip(S, C, OM, coff, ctag, lane, &newv);
         if (rc != CUT_OK) {
           status |= (rc == CUT_CAPACITY) ? TESS_CELL_CAPACITY : TESS_CELL_DEGENERATE;
           done = true;
@@ -622,7 +845,7 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
         if (ebase + lo >= e_stop) break;
       }
     }
-    __syncthreads();
+    wave_sync();
     if (stop_here || done) {
       e_stop_final = e_stop;
       done = true;
@@ -717,11 +940,11 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
     }
 #pragma unroll
     for (int r = 0; r < FPL; ++r)
-      if (fal[r]) S.fcontrib[rs[r]] = contrib[r];
-    __syncthreads();
+      if (fal[r]) S.u.fcontrib[rs[r]] = contrib[r];
+    wave_sync();
     if (lane == 0) {
       double vol = 0.0;
-      for (int f = 0; f < nfaces; ++f) vol = vol + S.fcontrib[f];
+      for (int f = 0; f < nfaces; ++f) vol = vol + S.u.fcontrib[f];
       A.volume[oi] = vol / 6.0;
       A.fcount[oi] = (uint32_t)nfaces;
     }
@@ -772,6 +995,15 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
       }
     }
     A.status[oi] = status;
+    if (A.summary) {
+      if (!(status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE))) {
+        atomicAdd(&A.summary->flops, (unsigned long long)(flops + oflops + 1));
+        atomicAdd(&A.summary->candidates, (unsigned long long)streamed);
+      }
+      if (status & TESS_CELL_DEGENERATE) atomicAdd(&A.summary->degenerate, 1u);
+      if (status & TESS_CELL_CAPACITY) atomicAdd(&A.summary->capacity, 1u);
+      if (status & TESS_CELL_TABLE) atomicAdd(&A.summary->table, 1u);
+    }
     if (A.stats) {
       tess_cell_stats st;
       st.streamed = streamed;

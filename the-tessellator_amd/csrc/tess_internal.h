@@ -26,6 +26,16 @@ __host__ __device__ inline void unpack_off(int32_t p, int& ox, int& oy, int& oz)
   oz = (p & 1023) - 512;
 }
 
+// Launch-wide counters accumulated by the cell kernel (one atomic per cell).
+struct Summary {
+  unsigned long long flops;       // algorithmic f64 ops of all cells
+  unsigned long long candidates;  // candidates streamed
+  unsigned int degenerate;        // cells with TESS_CELL_DEGENERATE
+  unsigned int capacity;          // cells with TESS_CELL_CAPACITY
+  unsigned int table;             // cells with TESS_CELL_TABLE
+  unsigned int fallback;          // learned mode: cells that fell back
+};
+
 // Faces staged per cell before CSR compaction.
 constexpr int kFaceStage = 64;
 
@@ -55,6 +65,7 @@ struct CellArgs {
   double* sarea;           // [n][kFaceStage]
   uint32_t* status;
   tess_cell_stats* stats;  // nullable
+  Summary* summary;        // nullable
   uint32_t* redo;          // grid positions of cells needing a larger variant
   uint32_t* nredo;
   // single-cell geometry export (nullable): vertices and face rings

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtess.so")
+LIB_PATH = os.environ.get("TESS_LIBRARY") or os.path.join(HERE, "libtess.so")
 DATA_PATH = os.path.join(HERE, "libtessdata.so")
 
 TESS_OK = 0
