@@ -434,12 +434,6 @@ void poly_rings(const Poly& P, std::vector<Ring>& rings) {
   }
 }
 
-double outputs(const Poly& P, std::vector<FaceOut>* faces, uint64_t* flops) {
-  std::vector<Ring> rings;
-  poly_rings(P, rings);
-  return ring_outputs(rings, faces, flops);
-}
-
 // ------------------------------------------------------- per-face clipping
 // The clip restated face by face.  Every face ring is clipped on its own
 // (vertices classified by vector3.rs:170-193 with tolerance 1e-12; Outside
@@ -1009,8 +1003,11 @@ struct orc_result {
 
 extern "C" {
 
+// expand_all_no_radius (celery.rs:971-1018) when max_radius is NaN, else
+// expand_all_in_radius (celery.rs:1023-1075): stop before the first entry
+// whose key (a squared distance) exceeds max_radius as given.
 size_t orc_expand_all(const double* x, const double* y, const double* z, size_t n, double density,
-                      double sxp, double syp, double szp, uint32_t* out) {
+                      double sxp, double syp, double szp, double max_radius, uint32_t* out) {
   Grid G;
   grid_setup(G, x, y, z, n, density, true);
   int64_t ix, iy, iz;
@@ -1018,6 +1015,7 @@ size_t orc_expand_all(const double* x, const double* y, const double* z, size_t 
   const int64_t cpd = G.g.cpd;
   size_t c = 0;
   for (size_t t = 0; t < G.key.size(); ++t) {
+    if (G.key[t] > max_radius) break;
     int64_t bx = ix + G.off[3 * t], by = iy + G.off[3 * t + 1], bz = iz + G.off[3 * t + 2];
     if (bx < 0 || bx >= cpd || by < 0 || by >= cpd || bz < 0 || bz >= cpd) continue;
     uint64_t b = (uint64_t)((bx * cpd + by) * cpd + bz);

@@ -65,9 +65,10 @@ size_t orc_search_order(int64_t cpd, double sx, double sy, double sz, int64_t L,
 /* generation index of an offset in the reference's loops (celery.rs:437-673) */
 uint64_t orc_generation_index(int64_t cpd, int32_t ox, int32_t oy, int32_t oz);
 
-/* expand_all_no_radius for a site: writes candidate point indices (all n). */
+/* Candidate stream of a site: expand_all_no_radius (max_radius NaN, all n
+ * points) or expand_all_in_radius(max_radius). Returns the count. */
 size_t orc_expand_all(const double* x, const double* y, const double* z, size_t n, double density,
-                      double sxp, double syp, double szp, uint32_t* out);
+                      double sxp, double syp, double szp, double max_radius, uint32_t* out);
 
 /* ---- single polyhedron (polyhedron.rs) ---- */
 typedef struct orc_poly orc_poly;

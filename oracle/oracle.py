@@ -64,7 +64,8 @@ def lib():
             "orc_grid_build": (None, [dp, dp, dp, sz, C.POINTER(Grid), dp, dp, dp]),
             "orc_search_order": (sz, [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, dp, dp, dp, dp]),
             "orc_generation_index": (C.c_uint64, [C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
-            "orc_expand_all": (sz, [dp, dp, dp, sz, C.c_double, C.c_double, C.c_double, C.c_double, dp]),
+            "orc_expand_all": (sz, [dp, dp, dp, sz, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    dp]),
             "orc_poly_new": (vp, [C.c_double] * 6),
             "orc_poly_new_faces": (vp, [C.c_double] * 6),
             "orc_poly_free": (None, [vp]),
@@ -176,11 +177,13 @@ def generation_index(cpd, ox, oy, oz):
     return lib().orc_generation_index(cpd, ox, oy, oz)
 
 
-def expand_all(xyz, site, density=1.25):
+def expand_all(xyz, site, density=1.25, max_radius=None):
+    """expand_all_no_radius (max_radius None) or expand_all_in_radius(max_radius)."""
     x, y, z = soa(xyz)
     out = np.empty(len(x), np.uint32)
+    r = float("nan") if max_radius is None else float(max_radius)
     m = lib().orc_expand_all(_p(x), _p(y), _p(z), len(x), density, float(site[0]), float(site[1]),
-                             float(site[2]), _p(out))
+                             float(site[2]), r, _p(out))
     return out[:m]
 
 
