@@ -887,10 +887,17 @@ struct FaceCell {  // per-face polygon clip
 };
 
 // compute_voronoi_cell (interface.rs:257-313) for point `i`.
+struct Filter {
+  double max_radius = std::numeric_limits<double>::quiet_NaN();  // expand_all_in_radius cut-off
+  const int64_t* groups = nullptr;                               // interface.rs:280-297
+  int64_t target = -1;
+};
+
 template <class Cell>
 void compute_cell(Grid& G, const double* x, const double* y, const double* z, size_t i,
                   const double lo[3], const double hi[3], int mode, Cell& C,
-                  std::vector<FaceOut>& faces, double* volume, orc_cell_stats* st, bool check) {
+                  std::vector<FaceOut>& faces, double* volume, orc_cell_stats* st, bool check,
+                  const Filter& flt) {
   const V3 site = {x[i], y[i], z[i]};
   C.init(lo, hi, site);
   uint64_t flops = 24 + 40;
@@ -933,7 +940,7 @@ void compute_cell(Grid& G, const double* x, const double* y, const double* z, si
       truncated = false;
       if (t >= m) break;
     }
-    if (!all && key[t] > stop_threshold(r2)) {
+    if ((!all && key[t] > stop_threshold(r2)) || key[t] > flt.max_radius) {
       s.stop_key = key[t];
       break;
     }
@@ -944,6 +951,7 @@ void compute_cell(Grid& G, const double* x, const double* y, const double* z, si
     for (uint64_t k = G.delim[b]; k < G.delim[b + 1]; ++k) {
       uint32_t j = G.sorted[k];
       if (j == i) continue;  // self (interface.rs:299-305, by internal index)
+      if (flt.groups && flt.target >= 0 && flt.groups[j] != flt.target) continue;
       s.streamed++;
       V3 p = {x[j] - site.x, y[j] - site.y, z[j] - site.z};
       double d2 = dot(p, p);
@@ -1143,6 +1151,18 @@ int orc_poly_outgoing(const orc_poly* p, const double p4[4], double src[3], doub
 orc_result* orc_compute(const double* x, const double* y, const double* z, size_t n,
                         const double lo[3], const double hi[3], double density, int mode, int clip,
                         const uint32_t* cells, size_t ncells, int nthreads) {
+  return orc_compute_query(x, y, z, n, lo, hi, density, mode, clip, cells, ncells, nthreads,
+                           std::numeric_limits<double>::quiet_NaN(), nullptr, -1);
+}
+
+orc_result* orc_compute_query(const double* x, const double* y, const double* z, size_t n,
+                              const double lo[3], const double hi[3], double density, int mode, int clip,
+                              const uint32_t* cells, size_t ncells, int nthreads, double max_radius,
+                              const int64_t* groups, int64_t target_group) {
+  Filter flt;
+  flt.max_radius = max_radius;
+  flt.groups = groups;
+  flt.target = target_group;
   orc_result* R = new orc_result;
   if (n == 0) return R;
   auto t0 = std::chrono::steady_clock::now();
@@ -1169,9 +1189,9 @@ orc_result* orc_compute(const double* x, const double* y, const double* z, size_
     for (size_t c = 0; c < ncells; ++c) {
       size_t i = cells ? cells[c] : c;
       if ((clip & 1) == ORC_CLIP_WALK)
-        compute_cell(G, x, y, z, i, lo, hi, mode, W, faces, &R->volume[c], &R->stats[c], check);
+        compute_cell(G, x, y, z, i, lo, hi, mode, W, faces, &R->volume[c], &R->stats[c], check, flt);
       else
-        compute_cell(G, x, y, z, i, lo, hi, mode, F, faces, &R->volume[c], &R->stats[c], check);
+        compute_cell(G, x, y, z, i, lo, hi, mode, F, faces, &R->volume[c], &R->stats[c], check, flt);
       per[c] = faces;
       R->fcount[c] = (uint32_t)faces.size();
     }

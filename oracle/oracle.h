@@ -120,6 +120,13 @@ typedef struct orc_result orc_result;
 orc_result* orc_compute(const double* x, const double* y, const double* z, size_t n,
                         const double lo[3], const double hi[3], double density, int mode, int clip,
                         const uint32_t* cells, size_t ncells, int nthreads);
+/* Same with the reference's query options: max_radius (NaN = None) stops the
+ * stream at the first shell key above it (celery.rs:1023-1075) and
+ * target_group >= 0 keeps only points of that group (interface.rs:280-297). */
+orc_result* orc_compute_query(const double* x, const double* y, const double* z, size_t n,
+                              const double lo[3], const double hi[3], double density, int mode, int clip,
+                              const uint32_t* cells, size_t ncells, int nthreads, double max_radius,
+                              const int64_t* groups, int64_t target_group);
 size_t orc_result_cells(const orc_result*);
 uint64_t orc_result_total_faces(const orc_result*);
 /* CSR copy; face_offset has ncells+1 entries. */

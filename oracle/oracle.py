@@ -77,6 +77,8 @@ def lib():
             "orc_poly_check": (C.c_int, [vp]),
             "orc_poly_outgoing": (C.c_int, [vp, dp, dp, dp]),
             "orc_compute": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int]),
+            "orc_compute_query": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int,
+                                       C.c_double, dp, C.c_int64]),
             "orc_result_cells": (sz, [vp]),
             "orc_result_total_faces": (C.c_uint64, [vp]),
             "orc_result_copy": (None, [vp, dp, dp, dp, dp, dp]),
@@ -231,14 +233,18 @@ class Poly:
 
 
 # ---- cells (interface.rs) ------------------------------------------------
-def compute(xyz, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, cells=None, threads=0):
+def compute(xyz, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, cells=None, threads=0,
+            max_radius=None, groups=None, target_group=None):
     """All (or the listed) cells: dict(volume, face_count, face_offset, neighbor, area, stats, seconds)."""
     x, y, z = soa(xyz)
     n = len(x)
     lo, hi = _v(lo), _v(hi)
     cl = None if cells is None else np.ascontiguousarray(cells, np.uint32)
     nc = n if cl is None else len(cl)
-    r = lib().orc_compute(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(cl), nc, threads)
+    g = None if groups is None else np.ascontiguousarray(groups, np.int64)
+    r = lib().orc_compute_query(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(cl), nc, threads,
+                                float("nan") if max_radius is None else float(max_radius), _p(g),
+                                -1 if target_group is None else int(target_group))
     try:
         f = lib().orc_result_total_faces(r)
         vol, fc = np.empty(nc), np.empty(nc, np.uint32)

@@ -10,6 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import weakref
 
 import numpy as np
 
@@ -129,6 +130,7 @@ class Context:
         if st != TESS_OK:
             raise TessError(st, f"cannot open HIP device {device}")
         self.h = h
+        self._diagrams = weakref.WeakSet()  # destroyed before the context
 
     def check(self, st):
         if st != TESS_OK:
@@ -146,8 +148,16 @@ class Context:
 
     def close(self):
         if self.h:
+            for d in list(self._diagrams):
+                d.close()
             self.lib.tess_ctx_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -220,11 +230,12 @@ class Diagram:
         ctx.check(self.lib.tess_diagram_create(ctx.h, _ptr(xyz), self.n, _ptr(lo), _ptr(hi), _ptr(g),
                                                C.byref(o), C.byref(h)))
         self.h = h
+        ctx._diagrams.add(self)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.lib.tess_diagram_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
