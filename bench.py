@@ -1,21 +1,32 @@
 #!/usr/bin/env python3
-"""Voronoi cells/sec on one MI355X.
+"""Voronoi cells/sec on one MI355X: plain vs learned-radius batched kernel.
 
 One step = one full tessellation of the workload with the points already in
 HBM: bounding box, device counting sort into the uniform grid, the cell
-kernel (every cell's volume, face count, face areas and sorted neighbour
+passes (every cell's volume, face count, face areas and sorted neighbour
 list), face offsets and CSR compaction.  Prints one JSON line.
 
 Workloads (--workload):
-  uniform1m    1,000,000 points i.i.d. uniform in [0,1]^3, seed 1, grid of
-               about 5 points per cell (cell_density 5), plain mode
-  clustered4m  4,000,000 points, 80% in 64 Gaussian clusters (sigma 0.02),
-               20% uniform, seed 25, cell_density 5
+  clustered4m  (default) 4,000,000 points in [0,1]^3, 80% from 64 Gaussian
+               clusters (sigma 0.02, resampled inside the box) + 20% uniform,
+               seed 25, grid of about 5 points per cell.  Timed with the
+               learned-radius batched path (tess_batch: graph-captured
+               binning and cell passes, learned cost order and capacity,
+               autotuned); the plain kernel (tess_diagram) is timed on the
+               same points in the same run.  The model is trained first on
+               the GPU from the plain kernel's own counters on five
+               independently seeded sets of this kind (seeds 20-24).
+  uniform1m    1,000,000 points i.i.d. uniform in [0,1]^3, seed 1, plain
+               kernel only.
+  batch512     512 independent sets, N log-uniform on [1e3, 2e5], box sides
+               uniform on [0.5, 2], uniform points (seeds 3000 + i); batched
+               path vs the plain kernel set by set.
 """
 import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -32,13 +43,21 @@ import tess  # noqa: E402
 # off for bit-exact parity), so every counted op is one instruction lane-op.
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+METRIC = "Voronoi cells/sec (f64, 1 MI355X): plain vs learned-radius batched kernel"
 
 WORKLOADS = {
+    "clustered4m": dict(desc="learned-radius batched kernel, 4,000,000 clustered points (64 Gaussians sigma 0.02 "
+                             "+ 20% uniform) in [0,1]^3, f64, box walls",
+                        points=4_000_000, seed=25, density=5.0, kind="clustered", train_seeds=(20, 21, 22, 23, 24)),
     "uniform1m": dict(desc="plain kernel, 1,000,000 points i.i.d. uniform in [0,1]^3, f64, box walls",
                       points=1_000_000, seed=1, density=5.0, kind="uniform"),
-    "clustered4m": dict(desc="plain kernel, 4,000,000 clustered points (64 Gaussians sigma 0.02 + 20% uniform)",
-                        points=4_000_000, seed=25, density=5.0, kind="clustered"),
+    "batch512": dict(desc="batched kernel, 512 uniform sets, N log-uniform [1e3, 2e5], box sides U[0.5, 2]",
+                     sets=512, seed=3000, density=5.0, kind="batch"),
 }
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def make_points(w):
@@ -47,25 +66,152 @@ def make_points(w):
     return tess.clustered(w["points"], w["seed"])
 
 
-def cpu_baseline(pts, density, sample, threads):
+def cpu_baseline(pts, lo, hi, density, sample, threads):
     """Time the CPU restatement (oracle) on a random sample of the same cells."""
     from oracle import oracle as orc
 
     rng = np.random.default_rng(0)
     cells = np.sort(rng.choice(len(pts), min(sample, len(pts)), replace=False)).astype(np.uint32)
-    r = orc.compute(pts, np.zeros(3), np.ones(3), density=density, cells=cells, threads=threads)
+    r = orc.compute(pts, lo, hi, density=density, cells=cells, threads=threads)
     return len(cells) / r["seconds"], r["seconds"], len(cells)
 
 
+def timed(fn, steps, warmup, ctx):
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    tims = []
+    for _ in range(steps):
+        tims.append(fn())
+    ctx.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, tims
+
+
+def mean(tims, k):
+    return float(np.mean([t[k] for t in tims]))
+
+
+def train_model(ctx, w, path):
+    """Train the radius model on the plain kernel's counters of the training sets."""
+    m = tess.Model(ctx, seed=7)
+    t0 = time.perf_counter()
+    for s in w["train_seeds"]:
+        pts = tess.clustered(w["points"], s)
+        d = tess.Diagram(ctx, pts, np.zeros(3), np.ones(3), density=w["density"], collect_stats=True)
+        d.compute()
+        m.add_samples(d)
+        d.close()
+        log(f"training samples from seed {s}: {m.sample_count()}")
+    t1 = time.perf_counter()
+    loss = m.train(epochs=3, batch=8192, lr=2e-3, seed=1)
+    t2 = time.perf_counter()
+    m.save(path)
+    log(f"model trained: loss per epoch {list(np.round(loss, 4))}")
+    return m, dict(samples=m.sample_count(), sample_s=t1 - t0, train_s=t2 - t1,
+                   loss=[float(x) for x in loss])
+
+
+def bench_clustered(ctx, w, args):
+    pts = make_points(w)
+    n = len(pts)
+    lo, hi = np.zeros(3), np.ones(3)
+    path = os.path.join(tempfile.gettempdir(), "tess_clustered4m_model.bin")
+    m, train = train_model(ctx, w, path)
+    # plain kernel on the same points
+    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"])
+
+    def plain_step():
+        d.compute()
+        return d.timings()
+
+    plain_ms, plain_t = timed(plain_step, max(2, args.steps // 2), 1, ctx)
+    plain_cells = d.cells()
+    log(f"plain: {plain_ms:.1f} ms/step")
+    # learned-radius batched path
+    b = tess.Batch(ctx, [(pts, lo, hi)], density=w["density"])
+    cache = os.path.join(tempfile.gettempdir(), "tess_tune_cache.txt")
+    cfg = b.tune(m, cache_path=cache)
+    log(f"tuned config: {cfg}")
+
+    def step():
+        b.compute(m)
+        return b.timings()
+
+    ms, tims = timed(step, args.steps, args.warmup, ctx)
+    got = b.cells(0)
+    identical = all(np.array_equal(got[k], plain_cells[k]) for k in plain_cells)
+    log(f"learned batched: {ms:.1f} ms/step, bitwise equal to plain: {identical}")
+    extra = {
+        "plain": {"value": n / (plain_ms / 1e3), "ms_per_step": plain_ms,
+                  "phases_ms": {k: mean(plain_t, k) for k in ("bin_ms", "cells_ms", "compact_ms")},
+                  "capacity_reruns": int(plain_t[-1]["capacity_reruns"])},
+        "speedup_vs_plain": plain_ms / ms,
+        "bitwise_equal_to_plain": bool(identical),
+        "fallback_cells": int(tims[-1]["fallback_cells"]),
+        "fallback_rate": tims[-1]["fallback_cells"] / n,
+        "tuned_config": cfg,
+        "model": train,
+    }
+    return pts, lo, hi, n, ms, tims, got, extra, {"mode": "learned-batched"}
+
+
+def bench_uniform(ctx, w, args):
+    pts = make_points(w)
+    n = len(pts)
+    lo, hi = np.zeros(3), np.ones(3)
+    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"], vertex_capacity=args.vcap)
+
+    def step():
+        d.compute()
+        return d.timings()
+
+    ms, tims = timed(step, args.steps, args.warmup, ctx)
+    return pts, lo, hi, n, ms, tims, d.cells(), {}, {"mode": "plain", "vertex_capacity": args.vcap or "auto"}
+
+
+def bench_batch(ctx, w, args):
+    sets = []
+    for i in range(w["sets"]):
+        p, side = tess.batch_set(w["seed"] + i)
+        sets.append((p, np.zeros(3), side))
+    n = sum(len(p) for p, _, _ in sets)
+    log(f"batch: {len(sets)} sets, {n} points")
+    b = tess.Batch(ctx, sets, density=w["density"])
+    cache = os.path.join(tempfile.gettempdir(), "tess_tune_cache.txt")
+    cfg = b.tune(None, cache_path=cache)
+
+    def step():
+        b.compute()
+        return b.timings()
+
+    ms, tims = timed(step, args.steps, args.warmup, ctx)
+    # plain kernel set by set on the same sets (diagrams created beforehand;
+    # one pass of compute calls, wall clock)
+    ds = [tess.Diagram(ctx, p, lo, hi, density=w["density"]) for p, lo, hi in sets]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for d in ds:
+        d.compute()
+    ctx.synchronize()
+    plain_s = time.perf_counter() - t0
+    for d in ds:
+        d.close()
+    extra = {"plain_set_by_set": {"value": n / plain_s, "seconds": plain_s}, "speedup_vs_plain": plain_s * 1e3 / ms,
+             "tuned_config": cfg}
+    lo, hi = sets[0][1], sets[0][2]
+    return sets[0][0], lo, hi, n, ms, tims, None, extra, {"mode": "batched", "sets": len(sets)}
+
+
 def main():
-    ap = argparse.ArgumentParser(description=__doc__)
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="uniform1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="clustered4m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--vcap", type=int, default=0, help="vertex capacity of the cell kernel (0: automatic)")
+    ap.add_argument("--vcap", type=int, default=0, help="vertex capacity of the plain kernel (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -73,72 +219,55 @@ def main():
         sys.exit(2)
 
     w = WORKLOADS[args.workload]
-    pts = make_points(w)
-    n = len(pts)
     ctx = tess.Context(0)
-    d = tess.Diagram(ctx, pts, np.zeros(3), np.ones(3), density=w["density"], vertex_capacity=args.vcap)
-    for _ in range(args.warmup):
-        d.compute()
-    ctx.synchronize()
-    cells_ms, bin_ms, compact_ms, flops = [], [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        d.compute()
-        t = d.timings()
-        cells_ms.append(t["cells_ms"])
-        bin_ms.append(t["bin_ms"])
-        compact_ms.append(t["compact_ms"])
-        flops.append(t["flops"])
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = n / (ms_per_step / 1e3)
-
-    kernel_s = float(np.mean(cells_ms)) / 1e3
-    achieved = float(np.mean(flops)) / kernel_s / 1e12
-    faces = int(d.cells()["face_count"].sum())
-    # algorithmic HBM bytes of the cell kernel per launch: site position + id
-    # (28 B), volume + face count + status (16 B), 12 B per staged face
-    alg_bytes = 44 * n + 12 * faces
-
+    fn = {"clustered": bench_clustered, "uniform": bench_uniform, "batch": bench_batch}[w["kind"]]
+    pts, lo, hi, n, ms, tims, cells, extra, cfg_extra = fn(ctx, w, args)
+    value = n / (ms / 1e3)
+    kernel_ms = mean(tims, "cells_ms")
+    flops = mean(tims, "flops")
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    faces = int(cells["face_count"].sum()) if cells is not None else None
     line = {
-        "metric": "Voronoi cells/sec (f64, 1 MI355X): plain vs learned-radius batched kernel",
+        "metric": METRIC,
         "value": value,
         "unit": "cells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded xoshiro256**)",
-        "config": {"workload": w["desc"], "points": n, "seed": w["seed"], "cell_density": w["density"],
-                   "mode": "plain", "box": [[0, 0, 0], [1, 1, 1]], "vertex_capacity": args.vcap or "auto"},
-        "phases_ms": {"bin": float(np.mean(bin_ms)), "cells": float(np.mean(cells_ms)),
-                      "compact": float(np.mean(compact_ms))},
+        "config": dict({"workload": w["desc"], "points": n, "cell_density": w["density"]}, **cfg_extra),
+        "phases_ms": {"bin": mean(tims, "bin_ms"), "plan": mean(tims, "plan_ms"), "cells": kernel_ms,
+                      "compact": mean(tims, "compact_ms")},
         "roofline": {
             "bound": "mfma",
             "ceiling": "FP64 (vector = matrix peak) 78.6 TFLOP/s, FMA counted as 2; the geometry issues "
-                       "only single-rounded add/sub/mul/div/sqrt, no MFMA",
-            "kernel": "k_cells (one cell per wavefront)",
+                       "only single-rounded add/sub/mul/div/sqrt on the vector units, no MFMA",
+            "kernel": "cell passes (first pass + capacity rerun + face scan), HIP events on the context stream",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "flops_per_launch": float(np.mean(flops)),
-            "kernel_ms": float(np.mean(cells_ms)),
-            "alg_hbm_gbs": alg_bytes / kernel_s / 1e9,
+            "flops_per_launch": flops,
+            "kernel_ms": kernel_ms,
             "traffic": None,
         },
     }
+    if faces is not None:
+        # algorithmic HBM bytes of the cell passes: site position + id (28 B),
+        # volume + face count + status (16 B), 12 B per staged face
+        line["roofline"]["alg_hbm_gbs"] = (44 * n + 12 * faces) / (kernel_ms / 1e3) / 1e9
+    line.update(extra)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        rate, sec, cnt = cpu_baseline(pts, w["density"], args.cpu_sample, threads)
+        rate, sec, cnt = cpu_baseline(pts, lo, hi, w["density"], args.cpu_sample, threads)
         line["cpu_baseline"] = {
             "value": rate, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"{cnt} random cells of the same {n}-point set (seed 0 choice), grid build included, "
+            "sample": f"{cnt} random cells of the {len(pts)}-point set (seed 0 choice), grid build included, "
                       f"{sec:.1f} s on {threads} threads of {os.cpu_count()} visible",
         }
     print(json.dumps(line), flush=True)

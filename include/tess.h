@@ -93,7 +93,7 @@ typedef struct {
   size_t n;
 } tess_cells_device;
 
-/* Phase timings of the last tess_diagram_compute (milliseconds, HIP events). */
+/* Phase timings of the last compute (milliseconds, HIP events). */
 typedef struct {
   double bin_ms;     /* bounds + counting sort + gather */
   double cells_ms;   /* cell kernel(s) */
@@ -103,6 +103,7 @@ typedef struct {
   uint64_t flops;    /* algorithmic f64 ops of the cells (collect_stats = 1) */
   uint64_t fallback_cells; /* learned mode: cells that fell back to the plain search */
   uint64_t capacity_reruns;/* cells rerun with a larger capacity */
+  double plan_ms;          /* learned mode: features + inference + work lists */
 } tess_timings;
 
 void tess_opts_default(tess_opts* opts);
@@ -174,6 +175,95 @@ tess_status tess_cell_face(const tess_cell* c, size_t face, double* area, int64_
 size_t tess_cell_face_vertices(const tess_cell* c, size_t face, double* xyz, size_t cap);
 /* Cell::compute_vertices (interface.rs:368-370), site-relative. */
 size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
+
+/* ---- learned search-radius model (no counterpart in the reference) -----
+ * A 27 -> 64 -> 64 -> 2 ReLU network.  Input: log(1 + count) of the 3x3x3
+ * grid cells around a point's grid cell (bf16, padded to TESS_MLP_IN).
+ * Outputs: log(R / s) and log(1 + C), where R is the cell's final security
+ * radius (max vertex distance), s the smallest grid-cell edge and C the
+ * number of candidates the plain search classifies; both are standardised
+ * with the mean/std kept in the model.  Weights are bf16 on the MFMA units
+ * with fp32 accumulation; an fp32 master copy is trained with Adam.
+ * Predictions only choose each cell's vertex capacity and its place in the
+ * work queue: results are identical to TESS_MODE_PLAIN for any prediction.
+ *
+ * Parameter vector (fp32, TESS_MLP_PARAMS): W1[32][64], W2[64][64],
+ * W3[64][16] (row-major [in][out]; columns 2..15 of W3 unused), b1[64],
+ * b2[64], b3[16].  norm[4] = mean0, std0, mean1, std1 of the targets.
+ * calib[4] = routing thresholds on log(1 + C): cells above calib[0] start
+ * at vertex capacity 128, above calib[1] at 256; calib[2..3] reserved. */
+#define TESS_MLP_IN 32
+#define TESS_MLP_PARAMS 7312
+typedef struct tess_model tess_model;
+
+tess_status tess_model_create(tess_ctx* ctx, uint64_t seed, tess_model** out);
+void tess_model_destroy(tess_model* m);
+tess_status tess_model_get(const tess_model* m, float* params, float norm[4], float calib[4]);
+tess_status tess_model_set(tess_model* m, const float* params, const float norm[4], const float calib[4]);
+tess_status tess_model_save(const tess_model* m, const char* path);
+tess_status tess_model_load(tess_model* m, const char* path);
+/* Append every cell of a diagram computed with collect_stats = 1 as a
+ * training sample (features on the device, targets from its counters). */
+tess_status tess_model_add_samples(tess_model* m, tess_diagram* d);
+size_t tess_model_sample_count(const tess_model* m);
+/* Standardise the targets, run `epochs` passes of Adam over the samples in
+ * shuffled mini-batches on the GPU, then set the routing thresholds from the
+ * samples' vertex counts.  loss_out[epochs]: mean loss per epoch (nullable). */
+tess_status tess_model_train(tess_model* m, int epochs, int batch, float lr, uint64_t seed, float* loss_out);
+/* Predictions for every point of a diagram (after compute or binning):
+ * out[n][2] = predicted R (length units) and C, in original order. */
+tess_status tess_diagram_predict(tess_diagram* d, const tess_model* m, float* out);
+/* All cells, learned schedule (same results as tess_diagram_compute). */
+tess_status tess_diagram_compute_learned(tess_diagram* d, const tess_model* m);
+/* All cells, schedule from caller-given predictions log(1 + C) per point
+ * (original order; any values, e.g. adversarial) and thresholds calib[2]. */
+tess_status tess_diagram_compute_scheduled(tess_diagram* d, const float* log1p_c, const float calib[2]);
+
+/* Low-level access for checking the network against a CPU reference.
+ * x: bf16 bit patterns [n][TESS_MLP_IN]; t: standardised targets [n][2]. */
+tess_status tess_diagram_features(tess_diagram* d, uint16_t* x);
+tess_status tess_mlp_forward(tess_model* m, const uint16_t* x, size_t n, float* y);
+/* Gradient of 0.5 * sum((y - t)^2) / n w.r.t. the parameters (bf16 weights
+ * as stored), summed over the batch in a fixed order; loss = that value. */
+tess_status tess_mlp_gradient(tess_model* m, const uint16_t* x, const float* t, size_t n, float* grad,
+                              float* loss);
+/* One 16x16x32 bf16 MFMA: d[16][16] = a[16][32] * b[32][16] (row-major). */
+tess_status tess_mfma_probe(tess_ctx* ctx, const uint16_t* a, const uint16_t* b, float* d);
+
+/* ---- batches of independent sets (no counterpart in the reference) -----
+ * Many point sets, each with its own container box and grid, tessellated
+ * together.  Per set the results are bitwise those of a tess_diagram of the
+ * same points.  The binning of every set is captured once into a HIP graph;
+ * the cells of all sets run as one list in one launch (plus a persistent
+ * rerun pass for cells that outgrow the first pass's vertex capacity),
+ * captured into a second graph that is replayed while the schedule is
+ * unchanged. */
+typedef struct tess_batch tess_batch;
+typedef struct {
+  int32_t first_capacity; /* vertex capacity of the first pass: 64, 128, 256 */
+  int32_t xcd_contiguous; /* 1: each XCD takes a contiguous range of the list */
+  int32_t redo_chunk;     /* cells a wavefront takes per grab in the rerun pass */
+  int32_t learned;        /* 1: heaviest predicted cells first (needs a model) */
+} tess_batch_config;
+
+/* xyz[s]: host AoS points of set s (n[s] of them); lo/hi: [nsets][3] boxes. */
+tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* xyz, const size_t* n,
+                              const double* lo, const double* hi, const tess_opts* opts, tess_batch** out);
+void tess_batch_destroy(tess_batch* b);
+size_t tess_batch_sets(const tess_batch* b);
+tess_status tess_batch_set_config(tess_batch* b, const tess_batch_config* cfg);
+tess_status tess_batch_get_config(const tess_batch* b, tess_batch_config* cfg);
+/* All cells of all sets; m may be NULL unless config.learned = 1. */
+tess_status tess_batch_compute(tess_batch* b, const tess_model* m);
+/* Time the candidate configurations on this batch and keep the fastest.
+ * cache_path (nullable): a text file of earlier results keyed by device and
+ * batch shape; a hit skips the timing, a miss appends a line. */
+tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cache_path, tess_batch_config* best);
+tess_status tess_batch_total_faces(const tess_batch* b, size_t set, uint64_t* out);
+/* One set's results, laid out as tess_diagram_copy_cells. */
+tess_status tess_batch_copy_cells(const tess_batch* b, size_t set, double* volume, uint32_t* face_count,
+                                  uint64_t* face_offset, int32_t* neighbor, double* face_area);
+tess_status tess_batch_timings(const tess_batch* b, tess_timings* out);
 
 /* ---- diagnostics (no counterpart in the reference) --------------------- */
 /* Device f64 sqrt, reciprocal and division of n inputs (for checking that

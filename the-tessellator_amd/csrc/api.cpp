@@ -21,35 +21,11 @@
 #include <tuple>
 #include <vector>
 
-#include "launch.h"
-#include "tess_internal.h"
+#include "runtime.h"
 
 using namespace tess;
 
-namespace {
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t alloc(size_t count) {
-    if (count <= n && p) return hipSuccess;
-    reset();
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)&p, sizeof(T) * count);
-    if (e == hipSuccess) n = count;
-    return e;
-  }
-};
+namespace tess {
 
 // ---------------------------------------------------------------- shell table
 // Generation order of celery.rs:437-673: the centre, then offsets with three,
@@ -83,14 +59,6 @@ double shell_key(int ox, int oy, int oz, double sx, double sy, double sz) {
                gz = oz ? std::abs(oz) - 1 : 0;
   return sq(gx * sx) + sq(gy * sy) + sq(gz * sz);
 }
-
-struct Table {
-  DevBuf<double> key;
-  DevBuf<int32_t> off;
-  int len = 0;
-  double kcut = 0;
-  int L = 0;
-};
 
 // Entries with |o|_inf <= L sorted by (key, generation index).  For L < cpd-1
 // only keys below kcut = min_a (L s_a)^2 are kept: every offset outside the
@@ -128,8 +96,6 @@ void build_table_host(int64_t cpd, double sx, double sy, double sz, int L, std::
   *kcut = kc;
 }
 
-constexpr int kTableL = 12;
-
 uint64_t bits(double v) {
   uint64_t u;
   std::memcpy(&u, &v, 8);
@@ -144,99 +110,9 @@ int64_t icbrt_floor(double x) {
   return k;
 }
 
-}  // namespace
+}  // namespace tess
 
-struct tess_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::mutex mu;
-  // shell tables keyed by (cpd, sx, sy, sz, L)
-  std::map<std::tuple<int64_t, uint64_t, uint64_t, uint64_t, int>, std::unique_ptr<Table>> tables;
-  DevBuf<double> diag;
-
-  tess_status fail(tess_status s, const std::string& m) {
-    err = m;
-    return s;
-  }
-  tess_status hip(hipError_t e, const char* what) {
-    if (e == hipSuccess) return TESS_OK;
-    err = std::string(what) + ": " + hipGetErrorString(e);
-    return TESS_E_DEVICE;
-  }
-  Table* table(int64_t cpd, double sx, double sy, double sz, int L, tess_status* st) {
-    auto key = std::make_tuple(cpd, bits(sx), bits(sy), bits(sz), L);
-    auto it = tables.find(key);
-    if (it != tables.end()) return it->second.get();
-    std::vector<double> k;
-    std::vector<int32_t> o;
-    double kc;
-    build_table_host(cpd, sx, sy, sz, L, k, o, &kc);
-    auto t = std::make_unique<Table>();
-    if (t->key.alloc(k.size()) != hipSuccess || t->off.alloc(o.size()) != hipSuccess) {
-      *st = fail(TESS_E_NOMEM, "shell table allocation failed");
-      return nullptr;
-    }
-    if (hipMemcpyAsync(t->key.p, k.data(), 8 * k.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipMemcpyAsync(t->off.p, o.data(), 4 * o.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess) {
-      *st = fail(TESS_E_DEVICE, "shell table upload failed");
-      return nullptr;
-    }
-    t->len = (int)k.size();
-    t->kcut = kc;
-    t->L = L;
-    Table* r = t.get();
-    tables[key] = std::move(t);
-    return r;
-  }
-};
-
-struct tess_cell {
-  int64_t original = -1;
-  double volume = 0;
-  std::vector<int64_t> nbr;
-  std::vector<double> area;
-  std::vector<std::vector<double>> ring;  // site-relative coordinates per face
-  std::vector<double> verts;
-};
-
-struct tess_diagram {
-  tess_ctx* ctx = nullptr;
-  tess_opts opts;
-  size_t n = 0;
-  int64_t cpd = 0;
-  int nbins = 0;
-  double lo[3], hi[3];
-  bool have_groups = false;
-  // inputs
-  DevBuf<double> x, y, z, xyz_tmp;
-  DevBuf<int64_t> groups;
-  // grid
-  DevBuf<GridDev> grid;
-  DevBuf<double> part;
-  DevBuf<uint32_t> bins, cnt, delim, cursor, sorted, scratch, sums, big, misc, inv;
-  DevBuf<double> sx, sy, sz;
-  GridDev hgrid;
-  bool binned = false;
-  bool computed = false;
-  bool inv_ready = false;
-  // outputs
-  DevBuf<double> volume, sarea, area;
-  DevBuf<uint32_t> fcount, status, redo;
-  DevBuf<int32_t> snbr, nbr;
-  DevBuf<uint64_t> foff, fsums;
-  DevBuf<tess_cell_stats> stats;
-  DevBuf<Summary> summary;
-  DevBuf<uint32_t> work1;
-  DevBuf<double> exp_v;
-  DevBuf<int32_t> exp_ring, exp_meta;
-  uint64_t total_faces = 0;
-  tess_timings tim;
-  uint64_t degenerate = 0;
-};
-
-namespace {
+namespace tess {
 
 tess_status alloc_all(tess_diagram* d) {
   tess_ctx* c = d->ctx;
@@ -261,7 +137,7 @@ tess_status alloc_all(tess_diagram* d) {
   return TESS_OK;
 }
 
-tess_status bin(tess_diagram* d) {
+tess_status bin_enqueue(tess_diagram* d) {
   tess_ctx* c = d->ctx;
   hipStream_t s = c->stream;
   const int n = (int)d->n;
@@ -275,9 +151,14 @@ tess_status bin(tess_diagram* d) {
   if ((st = c->hip(launch_counting_sort(d->x.p, d->y.p, d->z.p, n, d->grid.p, d->nbins, b, s), "sort")) !=
       TESS_OK)
     return st;
-  if ((st = c->hip(launch_gather(d->sorted.p, n, d->x.p, d->y.p, d->z.p, d->sx.p, d->sy.p, d->sz.p, s),
-                   "gather")) != TESS_OK)
-    return st;
+  return c->hip(launch_gather(d->sorted.p, n, d->x.p, d->y.p, d->z.p, d->sx.p, d->sy.p, d->sz.p, s), "gather");
+}
+
+tess_status bin(tess_diagram* d) {
+  tess_ctx* c = d->ctx;
+  hipStream_t s = c->stream;
+  tess_status st;
+  if ((st = bin_enqueue(d)) != TESS_OK) return st;
   if ((st = c->hip(hipMemcpyAsync(&d->hgrid, d->grid.p, sizeof(GridDev), hipMemcpyDeviceToHost, s),
                    "grid copy")) != TESS_OK)
     return st;
@@ -375,20 +256,135 @@ tess_status create_common(tess_ctx* ctx, size_t n, const double lo[3], const dou
   return TESS_OK;
 }
 
-tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, int nwork) {
+tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, int nwork, bool push_redo) {
   tess_ctx* c = d->ctx;
+  if (nwork <= 0) return TESS_OK;
   CellArgs a = base_args(d, t);
   a.work = work;
   a.nwork = nwork;
   a.xcd_remap = work ? 0 : 1;
-  a.redo = d->redo.p + 1;
-  a.nredo = d->redo.p;
-  tess_status st;
-  if ((st = c->hip(hipMemsetAsync(d->redo.p, 0, sizeof(uint32_t), c->stream), "memset")) != TESS_OK) return st;
+  a.redo = push_redo ? d->redo.p + 1 : nullptr;
+  a.nredo = push_redo ? d->redo.p : nullptr;
   return c->hip(launch_cells(vpl, a, c->stream), "cell kernel");
 }
 
+
+namespace {
+
+struct Events {
+  hipEvent_t e[5] = {};
+  bool ok = true;
+  Events() {
+    for (auto& x : e)
+      if (hipEventCreate(&x) != hipSuccess) ok = false;
+  }
+  ~Events() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  float ms(int a, int b) const {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e[a], e[b]);
+    return t;
+  }
+};
+
 }  // namespace
+
+tess_status compute_cells(tess_diagram* d, const Planner* plan) {
+  tess_ctx* c = d->ctx;
+  hipStream_t s = c->stream;
+  tess_status st;
+  Events ev;
+  if (!ev.ok) return c->fail(TESS_E_DEVICE, "event create failed");
+  (void)hipEventRecord(ev.e[0], s);
+  if ((st = bin(d)) != TESS_OK) return st;
+  (void)hipEventRecord(ev.e[1], s);
+  auto t0 = std::chrono::steady_clock::now();
+  const GridDev& G = d->hgrid;
+  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
+  auto t1 = std::chrono::steady_clock::now();
+  if (!t) return st;
+  const int n = (int)d->n;
+  // work lists: one pass over every cell at the default capacity, or the
+  // planner's lists, largest capacity first
+  Schedule sch;
+  if (plan) {
+    if ((st = (*plan)(d, sch)) != TESS_OK) return st;
+  } else {
+    sch.count[pick_vpl(d) >> 1] = n;  // vpl 1, 2, 4 -> slot 0, 1, 2
+  }
+  (void)hipEventRecord(ev.e[2], s);
+  if ((st = c->hip(hipMemsetAsync(d->summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK ||
+      (st = c->hip(hipMemsetAsync(d->redo.p, 0, sizeof(uint32_t), s), "memset")) != TESS_OK)
+    return st;
+  static const int kVpl[3] = {1, 2, 4};
+  for (int slot = 2; slot >= 0; --slot)
+    if (sch.count[slot] &&
+        (st = run_cells(d, t, kVpl[slot], sch.list[slot], sch.count[slot], slot < 2)) != TESS_OK)
+      return st;
+  // cells that outgrew their capacity: rerun with the largest variant
+  uint32_t nredo = 0;
+  Summary sm1;
+  if ((st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&sm1, d->summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
+          TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
+    return st;
+  if (nredo) {
+    if (d->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
+                     "redo")) != TESS_OK ||
+        (st = c->hip(hipMemsetAsync(&d->summary.p->capacity, 0, sizeof(unsigned int), s), "memset")) != TESS_OK ||
+        (st = run_cells(d, t, 4, d->redo_list.p, (int)nredo, false)) != TESS_OK)
+      return st;
+    if (plan &&
+        (st = c->hip(launch_mark_status(d->redo_list.p, (int)nredo, d->sorted.p, d->status.p,
+                                        d->opts.collect_stats ? d->stats.p : nullptr, TESS_CELL_FALLBACK, s),
+                     "mark")) != TESS_OK)
+      return st;
+  }
+  (void)hipEventRecord(ev.e[3], s);
+  if ((st = c->hip(launch_face_offsets(d->fcount.p, n, d->foff.p, d->fsums.p, s), "face offsets")) != TESS_OK)
+    return st;
+  uint64_t total = 0;
+  if ((st = c->hip(hipMemcpyAsync(&total, d->foff.p + n, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK)
+    return st;
+  if (d->nbr.alloc(total) != hipSuccess || d->area.alloc(total) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if ((st = c->hip(launch_compact(d->fcount.p, d->foff.p, n, d->snbr.p, d->sarea.p, d->nbr.p, d->area.p, s),
+                   "compact")) != TESS_OK)
+    return st;
+  (void)hipEventRecord(ev.e[4], s);
+  Summary sm2;
+  if ((st = c->hip(hipMemcpyAsync(&sm2, d->summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
+          TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "compute")) != TESS_OK)
+    return st;
+  d->tim.bin_ms = ev.ms(0, 1);
+  d->tim.plan_ms = ev.ms(1, 2);
+  d->tim.cells_ms = ev.ms(2, 3);
+  d->tim.compact_ms = ev.ms(3, 4);
+  d->tim.total_ms = ev.ms(0, 4);
+  d->tim.table_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  d->tim.capacity_reruns = nredo;
+  d->tim.fallback_cells = plan ? nredo : 0;
+  d->tim.flops = sm2.flops;
+  d->total_faces = total;
+  d->computed = true;
+  // every first-pass overflow that went to the redo list is counted again
+  // (or not) by the rerun; overflows of the largest variant are failures
+  const uint64_t cap = nredo ? (uint64_t)(sm1.capacity - nredo) + sm2.capacity : sm1.capacity;
+  const uint64_t bad = sm2.degenerate, tab = sm2.table;
+  d->degenerate = bad;
+  if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
+  if (tab) return c->fail(TESS_E_CAPACITY, std::to_string(tab) + " cells exhausted the shell table");
+  if (bad) return c->fail(TESS_E_DEGENERATE, std::to_string(bad) + " cells failed topology checks");
+  return TESS_OK;
+}
+
+}  // namespace tess
 
 extern "C" {
 
@@ -507,118 +503,9 @@ size_t tess_diagram_size(const tess_diagram* d) { return d ? d->n : 0; }
 
 tess_status tess_diagram_compute(tess_diagram* d) {
   if (!d) return TESS_E_INVALID;
-  tess_ctx* c = d->ctx;
-  std::lock_guard<std::mutex> g(c->mu);
-  (void)hipSetDevice(c->device);
-  hipStream_t s = c->stream;
-  tess_status st;
-  hipEvent_t ev[4];
-  for (auto& e : ev)
-    if (hipEventCreate(&e) != hipSuccess) return c->fail(TESS_E_DEVICE, "event create failed");
-  auto cleanup = [&]() {
-    for (auto& e : ev) (void)hipEventDestroy(e);
-  };
-  (void)hipEventRecord(ev[0], s);
-  if ((st = bin(d)) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  (void)hipEventRecord(ev[1], s);
-  if ((st = c->hip(hipStreamSynchronize(s), "binning")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  auto t0 = std::chrono::steady_clock::now();
-  const GridDev& G = d->hgrid;
-  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
-  auto t1 = std::chrono::steady_clock::now();
-  if (!t) {
-    cleanup();
-    return st;
-  }
-  const int n = (int)d->n;
-  const int vpl = pick_vpl(d);
-  if ((st = c->hip(hipMemsetAsync(d->summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  if ((st = run_cells(d, t, vpl, nullptr, n)) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  // cells that outgrew the capacity: rerun with the largest variant
-  uint32_t nredo = 0;
-  if ((st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  d->tim.capacity_reruns = nredo;
-  if (nredo && vpl < 4) {
-    DevBuf<uint32_t> list;
-    if (list.alloc(nredo) != hipSuccess) {
-      cleanup();
-      return c->fail(TESS_E_NOMEM, "device allocation failed");
-    }
-    (void)hipMemcpyAsync(list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s);
-    if ((st = run_cells(d, t, 4, list.p, (int)nredo)) != TESS_OK ||
-        (st = c->hip(hipStreamSynchronize(s), "cell kernel rerun")) != TESS_OK) {
-      cleanup();
-      return st;
-    }
-  }
-  (void)hipEventRecord(ev[2], s);
-  if ((st = c->hip(launch_face_offsets(d->fcount.p, n, d->foff.p, d->fsums.p, s), "face offsets")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  uint64_t total = 0;
-  if ((st = c->hip(hipMemcpyAsync(&total, d->foff.p + n, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  if (d->nbr.alloc(total) != hipSuccess || d->area.alloc(total) != hipSuccess) {
-    cleanup();
-    return c->fail(TESS_E_NOMEM, "device allocation failed");
-  }
-  if ((st = c->hip(launch_compact(d->fcount.p, d->foff.p, n, d->snbr.p, d->sarea.p, d->nbr.p, d->area.p, s),
-                   "compact")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  (void)hipEventRecord(ev[3], s);
-  if ((st = c->hip(hipStreamSynchronize(s), "compute")) != TESS_OK) {
-    cleanup();
-    return st;
-  }
-  float ms;
-  (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-  d->tim.bin_ms = ms;
-  (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
-  d->tim.cells_ms = ms;
-  (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
-  d->tim.compact_ms = ms;
-  (void)hipEventElapsedTime(&ms, ev[0], ev[3]);
-  d->tim.total_ms = ms;
-  d->tim.table_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  cleanup();
-  d->total_faces = total;
-  d->computed = true;
-  // launch summary: capacity cells that the rerun fixed were counted once
-  // by the first pass and not again by the rerun, so re-derive them.
-  Summary sm;
-  if ((st = c->hip(hipMemcpy(&sm, d->summary.p, sizeof(Summary), hipMemcpyDeviceToHost), "summary")) != TESS_OK)
-    return st;
-  d->tim.flops = sm.flops;
-  const uint64_t bad = sm.degenerate, tab = sm.table;
-  const bool reran = nredo && vpl < 4;
-  const uint64_t cap = reran ? sm.capacity - nredo : sm.capacity;
-  d->degenerate = bad;
-  if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
-  if (tab) return c->fail(TESS_E_CAPACITY, std::to_string(tab) + " cells exhausted the shell table");
-  if (bad) return c->fail(TESS_E_DEGENERATE, std::to_string(bad) + " cells failed topology checks");
-  return TESS_OK;
+  std::lock_guard<std::mutex> g(d->ctx->mu);
+  (void)hipSetDevice(d->ctx->device);
+  return compute_cells(d, nullptr);
 }
 
 tess_status tess_diagram_total_faces(const tess_diagram* d, uint64_t* out) {

@@ -218,6 +218,11 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, int m, const uint32_t* __
   if (i == 0 && last) *last = *total;
 }
 
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, int m, uint32_t* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = src[i];
+}
+
 __global__ void k_scatter(const uint32_t* __restrict__ bins, int n, uint32_t* __restrict__ cursor,
                           uint32_t* __restrict__ sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,23 +397,33 @@ hipError_t launch_bounds(const double* x, const double* y, const double* z, int 
 
 hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
                                 const GridDev* g, int nbins, const SortBufs& b, hipStream_t s) {
+  // fill kernels rather than memset nodes: this sequence may be captured
+  // into a graph
   hipError_t e;
-  if ((e = hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * (size_t)nbins, s)) != hipSuccess) return e;
-  if ((e = hipMemsetAsync(b.misc, 0, sizeof(uint32_t) * 4, s)) != hipSuccess) return e;
+  if ((e = launch_fill_u32(b.cnt, (size_t)nbins, 0u, s)) != hipSuccess) return e;
+  if ((e = launch_fill_u32(b.misc, 4, 0u, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt);
   const int nsb = blocks(nbins, 1024);
   hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, b.cnt, nbins, b.delim, b.sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, b.sums, nsb, b.misc);
   hipLaunchKernelGGL(k_scan_add, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sums,
                      b.misc, b.delim + nbins);
-  if ((e = hipMemcpyAsync(b.cursor, b.delim, sizeof(uint32_t) * (size_t)nbins, hipMemcpyDeviceToDevice,
-                          s)) != hipSuccess)
-    return e;
+  hipLaunchKernelGGL(k_copy_u32, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.cursor);
   hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, b.bins, n, b.cursor, b.sorted);
   hipLaunchKernelGGL(k_sort_cells, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sorted,
                      b.big, b.misc + 1);
   hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(256), 0, s, b.delim, b.big, b.misc + 1, b.sorted,
                      b.scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_u32(const uint32_t* in, int m, uint32_t* out, uint32_t* sums, uint32_t* total,
+                           hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  const int nsb = blocks(m, 1024);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, in, m, out, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, sums, nsb, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks(m, 256)), dim3(256), 0, s, out, m, sums, total, nullptr);
   return hipGetLastError();
 }
 

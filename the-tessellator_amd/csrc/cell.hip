@@ -182,13 +182,25 @@ struct Lds {
   uint16_t he_next[HC];
   uint16_t f_start[FC];
   uint16_t f_seq[FC];
-  uint16_t exit_of[VC];
   uint16_t fentry[FC];  // fast clip: the Outside-run entry half-edge of each face
   uint16_t fexit[FC];   // ... and its exit half-edge
-  uint16_t run_a[64];
+  // per-cut scratch of the general clip and of the fast clip (never live
+  // at the same time)
+  union {
+    struct {
+      uint16_t exit_of[VC];
+      uint16_t run_a[64];
+      uint8_t newv_of[HC];
+    } g;
+    struct {
+      uint16_t cpair[64];    // crossing pair k: 2 * pair + (half-edge 2p points Inside)
+      uint8_t vslot[64];     // k-th free vertex slot
+      uint8_t pslot[64];     // k-th free pair slot
+      uint8_t rank[PC];      // rank of a crossing pair
+    } f;
+  } cs;
   uint8_t he_tgt[HC];
   uint8_t he_face[HC];
-  uint8_t newv_of[HC];
   uint8_t vcls[VC];
   uint8_t fmark[FC];
 };
@@ -229,39 +241,43 @@ __device__ __forceinline__ int rank_multi(const uint64_t (&m)[W], int p) {
 // one.
 template <int VPL, int PPL, int FPL>
 __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C,
-                                                   const uint64_t (&OM)[VPL], double off, int32_t tag, int lane,
-                                                   int* newv) {
-  // per pair: targets and faces packed in one word, classes in two bytes
-  uint32_t pk[PPL];
-  uint32_t cls[PPL];
+                                         const uint64_t (&OM)[VPL], double off, int32_t tag, int lane,
+                                         int* newv) {
+  // 1. scan pairs; crossing pairs get a rank (prefix of the crossing ballot)
+  //    and are listed in rank order with their targets and faces
   uint64_t CM[PPL], PD[PPL];
   bool slow = false;
+  int before = 0;
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
     const int p = lane + 64 * r;
     const bool al = (C.PA[r] >> lane) & 1ull;
-    uint32_t t0 = 0, t1 = 0, f0 = 0, f1 = 0, c0 = C_IN, c1 = C_IN;
+    uint32_t c0 = C_IN, c1 = C_IN;
     if (al) {
-      t0 = S.he_tgt[2 * p];
-      t1 = S.he_tgt[2 * p + 1];
-      f0 = S.he_face[2 * p];
-      f1 = S.he_face[2 * p + 1];
-      c0 = S.vcls[t0];
-      c1 = S.vcls[t1];
+      c0 = S.vcls[S.he_tgt[2 * p]];
+      c1 = S.vcls[S.he_tgt[2 * p + 1]];
     }
-    pk[r] = t0 | (t1 << 8) | (f0 << 16) | (f1 << 24);
-    cls[r] = c0 | (c1 << 8);
     const bool o0 = c0 == C_OUT, o1 = c1 == C_OUT;
     const bool cross = al && ((o0 && c1 == C_IN) || (o1 && c0 == C_IN));
     slow = slow || (al && ((o0 && c1 == C_INC) || (o1 && c0 == C_INC)));
     CM[r] = ballot(cross);
     PD[r] = ballot(al && (o0 || o1) && !cross);
+    if (cross) {
+      const int k = before + mbcnt(CM[r], lane);
+      if (k < 64) {
+        S.cs.f.cpair[k] = (uint16_t)(2 * p + (o0 ? 0 : 1));
+        S.cs.f.rank[p] = (uint8_t)k;
+      }
+    }
+    before += popc(CM[r]);
   }
   if (ballot(slow)) return CUT_GENERAL;
-  const int ncross = popc_multi(CM);
+  const int ncross = before;
   if (ncross < 3) return CUT_GENERAL;
-  // new vertices go to slots that hold no vertex now, so they can be written
-  // before the clip commits; bridges may reuse the pairs that die
+  if (ncross > 64) return CUT_CAPACITY;
+  // 2. the first ncross free vertex slots (slots holding no vertex now, so
+  //    they can be written before the clip commits) and free pair slots
+  //    (pairs that die are free)
   uint64_t vfree[VPL], pfree[PPL];
 #pragma unroll
   for (int r = 0; r < VPL; ++r) vfree[r] = ~C.VA[r];
@@ -273,68 +289,70 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
   for (int r = FPL - 1; r >= 0; --r)
     if (~C.FA[r]) nf = 64 * r + ctz64(~C.FA[r]);
   if (nf < 0) return CUT_CAPACITY;
-
-  // crossing pairs: intersection(outside, inside) into new slots, face maps
+  {
+    int bv = 0;
 #pragma unroll
-  for (int r = 0; r < PPL; ++r) {
-    if ((CM[r] >> lane) & 1ull) {
-      const int p = lane + 64 * r;
-      const bool first_out = (cls[r] & 0xff) == C_OUT;  // half-edge 2p points at the Outside end
-      const int t0 = pk[r] & 0xff, t1 = (pk[r] >> 8) & 0xff;
-      const int f0 = (pk[r] >> 16) & 0xff, f1 = pk[r] >> 24;
-      const int o = first_out ? t0 : t1, i = first_out ? t1 : t0;
-      const double dO = S.vdot[o], dI = S.vdot[i];
-      const double t = (off - dO) / (dI - dO);
-      const double ox = S.vx[o], oy = S.vy[o], oz = S.vz[o];
-      const int sv = select_multi(vfree, rank_multi(CM, p));
-      S.vx[sv] = ox + (S.vx[i] - ox) * t;
-      S.vy[sv] = oy + (S.vy[i] - oy) * t;
-      S.vz[sv] = oz + (S.vz[i] - oz) * t;
-      const int hin = first_out ? 2 * p : 2 * p + 1;
-      S.fentry[first_out ? f0 : f1] = (uint16_t)hin;
-      S.fexit[first_out ? f1 : f0] = (uint16_t)(hin ^ 1);
+    for (int r = 0; r < VPL; ++r) {
+      const int k = bv + mbcnt(vfree[r], lane);
+      if (((vfree[r] >> lane) & 1ull) && k < ncross) S.cs.f.vslot[k] = (uint8_t)(lane + 64 * r);
+      bv += popc(vfree[r]);
+    }
+    int bp = 0;
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) {
+      const int k = bp + mbcnt(pfree[r], lane);
+      if (((pfree[r] >> lane) & 1ull) && k < ncross) S.cs.f.pslot[k] = (uint8_t)(lane + 64 * r);
+      bp += popc(pfree[r]);
     }
   }
   wave_sync();
-  // one run per face: every writer must read back its own half-edge
+  // 3. lane k < ncross handles crossing pair k: intersection(outside,
+  //    inside) into new vertex slot k, the face maps
+  const bool mine = lane < ncross;
+  int fin = 0, fout = 0, hin = 0, sv = 0, sp = 0;
+  if (mine) {
+    hin = S.cs.f.cpair[lane];  // the half-edge that enters the Outside end
+    sv = S.cs.f.vslot[lane];
+    sp = S.cs.f.pslot[lane];
+    const int o = S.he_tgt[hin], i = S.he_tgt[hin ^ 1];
+    fin = S.he_face[hin];
+    fout = S.he_face[hin ^ 1];
+    const double dO = S.vdot[o], dI = S.vdot[i];
+    const double t = (off - dO) / (dI - dO);
+    const double ox = S.vx[o], oy = S.vy[o], oz = S.vz[o];
+    S.vx[sv] = ox + (S.vx[i] - ox) * t;
+    S.vy[sv] = oy + (S.vy[i] - oy) * t;
+    S.vz[sv] = oz + (S.vz[i] - oz) * t;
+    S.fentry[fin] = (uint16_t)hin;
+    S.fexit[fout] = (uint16_t)(hin ^ 1);
+  }
+  wave_sync();
+  // 4. one run per face: every writer reads back its own half-edge; the same
+  //    round trip fetches the run's exit (on its entry face) and the entry of
+  //    the run on its exit face
   bool clash = false;
-#pragma unroll
-  for (int r = 0; r < PPL; ++r) {
-    if ((CM[r] >> lane) & 1ull) {
-      const int p = lane + 64 * r;
-      const bool first_out = (cls[r] & 0xff) == C_OUT;
-      const int f0 = (pk[r] >> 16) & 0xff, f1 = pk[r] >> 24;
-      const int hin = first_out ? 2 * p : 2 * p + 1;
-      clash = clash || S.fentry[first_out ? f0 : f1] != hin || S.fexit[first_out ? f1 : f0] != (hin ^ 1);
-    }
+  int b = 0, a2 = 0;
+  if (mine) {
+    const int e_in = S.fentry[fin], x_out = S.fexit[fout];
+    b = S.fexit[fin];
+    a2 = S.fentry[fout];
+    clash = e_in != hin || x_out != (hin ^ 1);
   }
   if (ballot(clash)) return CUT_GENERAL;
-  // commit: bridge k on the entry face of crossing pair k, its flip on the new face
-#pragma unroll
-  for (int r = 0; r < PPL; ++r) {
-    if ((CM[r] >> lane) & 1ull) {
-      const int p = lane + 64 * r;
-      const bool first_out = (cls[r] & 0xff) == C_OUT;
-      const int f0 = (pk[r] >> 16) & 0xff, f1 = pk[r] >> 24;
-      const int fin = first_out ? f0 : f1, fout = first_out ? f1 : f0;
-      const int hin = first_out ? 2 * p : 2 * p + 1;
-      const int b = S.fexit[fin];    // exit of the run on the entry face
-      const int a2 = S.fentry[fout]; // entry of the run on the exit face
-      const int k = rank_multi(CM, p);
-      const int E = select_multi(vfree, k);
-      const int Xe = select_multi(vfree, rank_multi(CM, b >> 1));
-      const int br = 2 * select_multi(pfree, k);
-      const int nxt = 2 * select_multi(pfree, rank_multi(CM, a2 >> 1)) + 1;
-      S.he_tgt[hin] = (uint8_t)E;
-      S.he_next[hin] = (uint16_t)br;
-      S.he_tgt[br] = (uint8_t)Xe;
-      S.he_face[br] = (uint8_t)fin;
-      S.he_next[br] = (uint16_t)b;
-      S.he_tgt[br + 1] = (uint8_t)E;
-      S.he_face[br + 1] = (uint8_t)nf;
-      S.he_next[br + 1] = (uint16_t)nxt;
-      S.f_start[fin] = (uint16_t)br;
-    }
+  // 5. commit: bridge k on the entry face, its flip on the new face
+  if (mine) {
+    const int Xe = S.cs.f.vslot[S.cs.f.rank[b >> 1]];
+    const int nxt = 2 * S.cs.f.pslot[S.cs.f.rank[a2 >> 1]] + 1;
+    const int br = 2 * sp;
+    S.he_tgt[hin] = (uint8_t)sv;
+    S.he_next[hin] = (uint16_t)br;
+    S.he_tgt[br] = (uint8_t)Xe;
+    S.he_face[br] = (uint8_t)fin;
+    S.he_next[br] = (uint16_t)b;
+    S.he_tgt[br + 1] = (uint8_t)sv;
+    S.he_face[br + 1] = (uint8_t)nf;
+    S.he_next[br + 1] = (uint16_t)nxt;
+    S.f_start[fin] = (uint16_t)br;
   }
   uint64_t vnew[VPL], pnew[PPL];
   lowest_bits(vfree, ncross, vnew);
@@ -346,7 +364,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
   if (lane == 0) {
     S.f_tag[nf] = tag;
     S.f_seq[nf] = (uint16_t)C.seq;
-    S.f_start[nf] = (uint16_t)(2 * select_multi(pfree, 0) + 1);
+    S.f_start[nf] = (uint16_t)(2 * S.cs.f.pslot[0] + 1);
   }
   C.seq++;
   // faces whose every vertex was Outside (only possible when pairs died)
@@ -359,10 +377,10 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     wave_sync();
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
-      const int p = lane + 64 * r;
+      const int q = lane + 64 * r;
       if ((C.PA[r] >> lane) & 1ull) {
-        S.fmark[S.he_face[2 * p]] = 1;
-        S.fmark[S.he_face[2 * p + 1]] = 1;
+        S.fmark[S.he_face[2 * q]] = 1;
+        S.fmark[S.he_face[2 * q + 1]] = 1;
       }
     }
     wave_sync();
@@ -392,7 +410,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
                     double off, int32_t tag, int lane, int* newv) {
   constexpr int HC = Lds<VPL, PPL, FPL>::HC;
 #pragma unroll
-  for (int r = 0; r < VPL; ++r) S.exit_of[lane + 64 * r] = 0xffff;
+  for (int r = 0; r < VPL; ++r) S.cs.g.exit_of[lane + 64 * r] = 0xffff;
 
   // 1. scan pairs: Outside-run entries (P -> O, P not Outside) and pairs to
   //    delete (an Outside end whose other end is not Inside).
@@ -419,7 +437,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
     PD[r] = ballot(del);
     const uint64_t em = ballot(ent);
     const int pos = nruns + mbcnt(em, lane);
-    if (ent && pos < 64) S.run_a[pos] = (uint16_t)eh;
+    if (ent && pos < 64) S.cs.g.run_a[pos] = (uint16_t)eh;
     nruns += popc(em);
   }
   if (nruns > 64) return CUT_CAPACITY;
@@ -432,7 +450,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
   uint8_t cP = C_IN, cQ = C_IN;
   bool one = false, two = false, bad = false;
   if (isrun) {
-    a = S.run_a[lane];
+    a = S.cs.g.run_a[lane];
     P = S.he_tgt[a ^ 1];
     cP = S.vcls[P];
     int x = S.he_next[a], g = 0;
@@ -510,11 +528,11 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
     S.vx[vslot] = X0;
     S.vy[vslot] = X1;
     S.vz[vslot] = X2;
-    S.newv_of[b] = (uint8_t)vslot;
+    S.cs.g.newv_of[b] = (uint8_t)vslot;
   }
   wave_sync();
   const int Xid = needX ? vslot : Q;
-  const int Eid = isrun ? ((cP == C_IN) ? (int)S.newv_of[a ^ 1] : P) : 0;
+  const int Eid = isrun ? ((cP == C_IN) ? (int)S.cs.g.newv_of[a ^ 1] : P) : 0;
 
   // 5. rewire: bridge on the run's face, its flip on the new face.
   int nfh = -1;
@@ -532,17 +550,17 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
       S.he_next[pred] = (uint16_t)br;
     }
     S.f_start[fa] = (uint16_t)br;
-    S.exit_of[Xid] = (uint16_t)brf;
+    S.cs.g.exit_of[Xid] = (uint16_t)brf;
     nfh = brf;
   } else if (two) {
     S.he_face[nb] = (uint8_t)nf;
-    S.exit_of[Q] = (uint16_t)nb;
+    S.cs.g.exit_of[Q] = (uint16_t)nb;
     nfh = nb;
   }
   wave_sync();
   bool lbad = false;
   if (nfh >= 0) {
-    const int nx = S.exit_of[Eid];
+    const int nx = S.cs.g.exit_of[Eid];
     if (nx == 0xffff)
       lbad = true;
     else
@@ -609,21 +627,14 @@ __device__ double cell_r2(const CellState<VPL, PPL, FPL>& C, int lane) {
 
 }  // namespace
 
+// One cell: the site at grid position gpos of the set A.  Returns the
+// cell's status bits (wave-uniform); the caller queues capacity reruns.
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) void k_cells(CellArgs A) {
+__device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos, Lds<VPL, PPL, FPL>& S,
+                                              const int lane) {
   using L = Lds<VPL, PPL, FPL>;
   constexpr int FC = L::FC;
-  __shared__ L S;
   CellState<VPL, PPL, FPL> C;
-  const int lane = threadIdx.x;
-
-  int w = blockIdx.x;
-  if (A.xcd_remap) {
-    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
-    w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
-  if (w >= A.nwork) return;
-  const int gpos = A.work ? (int)A.work[w] : w;
   const uint32_t self = A.sorted[gpos];
   const double sx = A.spx[gpos], sy = A.spy[gpos], sz = A.spz[gpos];
   const GridDev G = *A.grid;
@@ -989,10 +1000,6 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
     if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) {
       A.volume[oi] = 0.0;
       A.fcount[oi] = 0;
-      if ((status & TESS_CELL_CAPACITY) && A.redo) {
-        const uint32_t slot = atomicAdd(A.nredo, 1u);
-        A.redo[slot] = (uint32_t)gpos;
-      }
     }
     A.status[oi] = status;
     if (A.summary) {
@@ -1018,11 +1025,86 @@ __global__ __launch_bounds__(64) void k_cells(CellArgs A) {
     }
   }
   (void)FC;
+  return status;
+}
+
+// One cell per wavefront over a set's grid positions (all, or a work list).
+template <int VPL, int PPL, int FPL>
+__global__ __launch_bounds__(64) void k_cells(CellArgs A) {
+  __shared__ Lds<VPL, PPL, FPL> S;
+  const int lane = threadIdx.x;
+  int w = blockIdx.x;
+  if (A.xcd_remap) {
+    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  if (w >= A.nwork) return;
+  const int gpos = A.work ? (int)A.work[w] : w;
+  const uint32_t status = cell_body(A, gpos, S, lane);
+  if (lane == 0 && (status & TESS_CELL_CAPACITY) && A.redo) {
+    const uint32_t slot = atomicAdd(A.nredo, 1u);
+    A.redo[slot] = (uint32_t)gpos;
+  }
+}
+
+// Cells of many sets in one launch.  An item is (set << 32) | grid position;
+// the set's arguments are read through the constant address space, as
+// scalar loads the compiler may repeat instead of keeping live.
+template <int VPL, int PPL, int FPL>
+__device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<VPL, PPL, FPL>& S, int lane) {
+  typedef const __attribute__((address_space(4))) CellArgs ConstArgs;
+  ConstArgs* cs = (ConstArgs*)(B.sets) + (uint32_t)(it >> 32);
+  const CellArgs& A = *(const CellArgs*)cs;
+  const uint32_t status = cell_body(A, (int)(uint32_t)it, S, lane);
+  if (lane == 0 && (status & TESS_CELL_CAPACITY) && B.redo) {
+    const uint32_t slot = atomicAdd(B.nredo, 1u);
+    B.redo[slot] = it;
+  }
+}
+
+// One wavefront per item of a list whose length the host knows (the grid):
+// the hardware dispatcher hands out the items in list order as wavefront
+// slots free up, which is the work queue of the pass.
+template <int VPL, int PPL, int FPL>
+__global__ __launch_bounds__(64) void k_cells_items(BatchArgs B, uint32_t n) {
+  __shared__ Lds<VPL, PPL, FPL> S;
+  uint32_t i = blockIdx.x;
+  if (B.xcd_remap) {
+    const uint32_t per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    i = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  if (i >= n) return;
+  batch_item(B, B.items[B.order ? B.order[i] : i], S, threadIdx.x);
+}
+
+// Persistent form for a list whose length is only known on the device
+// (*B.count): every wavefront takes `chunk` items at a time from a shared
+// counter until the list is exhausted, so the loop ends for every wavefront.
+template <int VPL, int PPL, int FPL>
+__global__ __launch_bounds__(64) void k_cells_queue(BatchArgs B) {
+  __shared__ Lds<VPL, PPL, FPL> S;
+  const int lane = threadIdx.x;
+  const uint32_t n = *B.count;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(B.counter, (uint32_t)B.chunk);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if (base >= n) break;
+    const uint32_t end = base + (uint32_t)B.chunk < n ? base + (uint32_t)B.chunk : n;
+    for (uint32_t i = base; i < end; ++i) {
+      wave_sync();
+      batch_item(B, B.items[B.order ? B.order[i] : i], S, lane);
+    }
+  }
 }
 
 template __global__ void k_cells<1, 2, 1>(CellArgs);
 template __global__ void k_cells<2, 3, 2>(CellArgs);
 template __global__ void k_cells<4, 6, 2>(CellArgs);
+template __global__ void k_cells_items<1, 2, 1>(BatchArgs, uint32_t);
+template __global__ void k_cells_items<2, 3, 2>(BatchArgs, uint32_t);
+template __global__ void k_cells_items<4, 6, 2>(BatchArgs, uint32_t);
+template __global__ void k_cells_queue<4, 6, 2>(BatchArgs);
 
 // f64 sqrt / reciprocal / division as compiled for the cell kernel.
 __global__ void k_diag_arith(const double* a, const double* b, int n, double* s, double* r, double* q) {
@@ -1036,6 +1118,37 @@ __global__ void k_diag_arith(const double* a, const double* b, int n, double* s,
 }  // namespace tess
 
 namespace tess {
+
+hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t nb = b.xcd_remap ? (n + 7) & ~7u : n;
+  switch (vpl) {
+    case 1:
+      hipLaunchKernelGGL((k_cells_items<1, 2, 1>), dim3(nb), dim3(64), 0, s, b, n);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_cells_items<2, 3, 2>), dim3(nb), dim3(64), 0, s, b, n);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_cells_items<4, 6, 2>), dim3(nb), dim3(64), 0, s, b, n);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cells_queue(const BatchArgs& b, int grid, hipStream_t s) {
+  if (grid <= 0 || b.chunk <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_cells_queue<4, 6, 2>), dim3(grid), dim3(64), 0, s, b);
+  return hipGetLastError();
+}
+
+int cells_queue_blocks_per_cu() {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cells_queue<4, 6, 2>, 64, 0);
+  return e == hipSuccess && nb > 0 ? nb : 1;
+}
 
 hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
   if (a.nwork <= 0) return hipSuccess;

@@ -22,6 +22,9 @@ struct SortBufs {
 };
 hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
                                 const GridDev* g, int nbins, const SortBufs& b, hipStream_t s);
+// exclusive scan of m u32 (sums: ceil(m / 1024) + 1 entries); *total = sum
+hipError_t launch_scan_u32(const uint32_t* in, int m, uint32_t* out, uint32_t* sums, uint32_t* total,
+                           hipStream_t s);
 hipError_t launch_gather(const uint32_t* sorted, int n, const double* x, const double* y,
                          const double* z, double* sx, double* sy, double* sz, hipStream_t s);
 hipError_t launch_face_offsets(const uint32_t* fcount, int n, uint64_t* foff, uint64_t* sums,
@@ -32,7 +35,52 @@ hipError_t launch_invert(const uint32_t* sorted, int n, uint32_t* inv, hipStream
 
 // cell.hip: vpl = vertex slots per lane (1, 2 or 4)
 hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s);
+// many sets: one wavefront per item of a list of host-known length n ...
+hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream_t s);
+// ... or persistent over a device-counted list at the largest capacity
+// (grid = resident blocks: cells_queue_blocks_per_cu() x CUs)
+hipError_t launch_cells_queue(const BatchArgs& b, int grid, hipStream_t s);
+int cells_queue_blocks_per_cu();
 hipError_t launch_diag_arith(const double* a, const double* b, int n, double* sq, double* rc,
                              double* dv, hipStream_t s);
+
+// sched.hip: work lists for the learned schedule
+constexpr int kSchedBuckets = 64;             // cost buckets per capacity slot
+constexpr int kSchedKeys = 3 * kSchedBuckets;  // slot-major keys
+// Key of every grid position from a per-point cost v = y[o * stride] * scale
+// + shift (log(1 + candidates)), o = sorted[k] (k when sorted is null):
+// capacity slot by thresholds t1 < t2, then
+// descending cost.  Writes the stable key order to list[n] and the start of
+// each slot (and n) to bounds[4].  hist: kSchedKeys * ceil(n / 256) entries;
+// sums: ceil(hist / 1024) + 1.
+hipError_t launch_schedule(const float* y, int stride, float scale, float shift, float t1, float t2,
+                           const uint32_t* sorted, int n, uint8_t* key, uint32_t* hist, uint32_t* sums,
+                           uint32_t* list, uint32_t* bounds, hipStream_t s);
+// batches: item list, per-item costs from per-point predictions, binning check
+hipError_t launch_batch_items(const uint64_t* base, int nsets, uint64_t total, uint64_t* items, hipStream_t s);
+hipError_t launch_batch_costs(const uint64_t* items, uint64_t total, const uint64_t* base, const uint32_t* sorted,
+                              const float* y, int stride, float scale, float shift, float* cost, hipStream_t s);
+hipError_t launch_batch_check(const uint32_t* delim, const uint64_t* dend, const uint64_t* base, int nsets,
+                              uint32_t* bad, hipStream_t s);
+// status[o] |= flag (and stats[o].status when stats is not null), o = sorted[list[i]]
+hipError_t launch_mark_status(const uint32_t* list, int m, const uint32_t* sorted, uint32_t* status,
+                              tess_cell_stats* stats, uint32_t flag, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t m, uint32_t v, hipStream_t s);
+
+// mlp.hip: learned search-radius model (bf16 MFMA, fp32 accumulate)
+hipError_t launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);
+hipError_t launch_features(const uint32_t* bins, const uint32_t* delim, int cpd, int n, uint16_t* feat,
+                           hipStream_t s);
+hipError_t launch_mlp_forward(const uint16_t* X, int n, const uint16_t* W, const float* bias, float* out,
+                              hipStream_t s);
+// one gradient slot per 32 samples: grad[(nsamp + 31) / 32][kMlpParams]
+hipError_t launch_mlp_train_step(const uint16_t* X, const float* T, const int* idx, int nsamp, float scale,
+                                 const uint16_t* W, const float* bias, float* grad, float* loss, hipStream_t s);
+hipError_t launch_targets(const tess_cell_stats* st, int n, double smin, float* T, uint16_t* vmax, hipStream_t s);
+hipError_t launch_grad_sum(const float* grad, int nslots, float* out, hipStream_t s);
+hipError_t launch_standardise(const float* T, size_t n, const float norm[4], float* Tn, hipStream_t s);
+hipError_t launch_weights_from_master(const float* master, uint16_t* wbf, float* bias, hipStream_t s);
+hipError_t launch_adam(const float* grad, int nslots, float* master, float* m1, float* m2, uint16_t* wbf,
+                       float* bias, float lr, float b1c, float b2c, hipStream_t s);
 
 }  // namespace tess

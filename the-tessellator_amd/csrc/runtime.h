@@ -1,0 +1,209 @@
+// runtime.h -- host-side objects behind include/tess.h, shared by the
+// translation units of the runtime (api.cpp, model.cpp, batch.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "launch.h"
+#include "tess_internal.h"
+
+namespace tess {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  bool own = true;  // false: a view into memory owned elsewhere
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p && own) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    own = true;
+  }
+  void view(T* q, size_t count) {
+    reset();
+    p = q;
+    n = count;
+    own = false;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    reset();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * count);
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+};
+
+struct Table {
+  DevBuf<double> key;
+  DevBuf<int32_t> off;
+  int len = 0;
+  double kcut = 0;
+  int L = 0;
+};
+
+// Shell table of a grid shape: entries with |o|_inf <= L in canonical order
+// (celery.rs:418-679), see build_table_host.
+void build_table_host(int64_t cpd, double sx, double sy, double sz, int L, std::vector<double>& key,
+                      std::vector<int32_t>& off, double* kcut);
+constexpr int kTableL = 12;
+uint64_t bits(double v);
+int64_t icbrt_floor(double x);
+
+}  // namespace tess
+
+using namespace tess;
+
+struct tess_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mu;
+  // shell tables keyed by (cpd, sx, sy, sz, L)
+  std::map<std::tuple<int64_t, uint64_t, uint64_t, uint64_t, int>, std::unique_ptr<Table>> tables;
+  DevBuf<double> diag;
+
+  tess_status fail(tess_status s, const std::string& m) {
+    err = m;
+    return s;
+  }
+  tess_status hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return TESS_OK;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return TESS_E_DEVICE;
+  }
+  Table* table(int64_t cpd, double sx, double sy, double sz, int L, tess_status* st) {
+    auto key = std::make_tuple(cpd, bits(sx), bits(sy), bits(sz), L);
+    auto it = tables.find(key);
+    if (it != tables.end()) return it->second.get();
+    std::vector<double> k;
+    std::vector<int32_t> o;
+    double kc;
+    build_table_host(cpd, sx, sy, sz, L, k, o, &kc);
+    auto t = std::make_unique<Table>();
+    if (t->key.alloc(k.size()) != hipSuccess || t->off.alloc(o.size()) != hipSuccess) {
+      *st = fail(TESS_E_NOMEM, "shell table allocation failed");
+      return nullptr;
+    }
+    if (hipMemcpyAsync(t->key.p, k.data(), 8 * k.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(t->off.p, o.data(), 4 * o.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      *st = fail(TESS_E_DEVICE, "shell table upload failed");
+      return nullptr;
+    }
+    t->len = (int)k.size();
+    t->kcut = kc;
+    t->L = L;
+    Table* r = t.get();
+    tables[key] = std::move(t);
+    return r;
+  }
+};
+
+struct tess_cell {
+  int64_t original = -1;
+  double volume = 0;
+  std::vector<int64_t> nbr;
+  std::vector<double> area;
+  std::vector<std::vector<double>> ring;  // site-relative coordinates per face
+  std::vector<double> verts;
+};
+
+struct tess_diagram {
+  tess_ctx* ctx = nullptr;
+  tess_opts opts;
+  size_t n = 0;
+  int64_t cpd = 0;
+  int nbins = 0;
+  double lo[3], hi[3];
+  bool have_groups = false;
+  // inputs
+  DevBuf<double> x, y, z, xyz_tmp;
+  DevBuf<int64_t> groups;
+  // grid
+  DevBuf<GridDev> grid;
+  DevBuf<double> part;
+  DevBuf<uint32_t> bins, cnt, delim, cursor, sorted, scratch, sums, big, misc, inv;
+  DevBuf<double> sx, sy, sz;
+  GridDev hgrid;
+  bool binned = false;
+  bool computed = false;
+  bool inv_ready = false;
+  // outputs
+  DevBuf<double> volume, sarea, area;
+  DevBuf<uint32_t> fcount, status, redo;
+  DevBuf<int32_t> snbr, nbr;
+  DevBuf<uint64_t> foff, fsums;
+  DevBuf<tess_cell_stats> stats;
+  DevBuf<Summary> summary;
+  DevBuf<uint32_t> work1, redo_list;
+  // learned schedule: features, predictions, work lists
+  DevBuf<uint16_t> feat;
+  DevBuf<float> pred;
+  DevBuf<uint32_t> sched_key, sched_list, sched_hist, sched_sums;
+  DevBuf<double> exp_v;
+  DevBuf<int32_t> exp_ring, exp_meta;
+  uint64_t total_faces = 0;
+  tess_timings tim;
+  uint64_t degenerate = 0;
+};
+
+#include <limits>
+
+struct tess_model {
+  tess_ctx* ctx = nullptr;
+  DevBuf<float> master, m1, m2, bias;
+  DevBuf<uint16_t> wbf;
+  float norm[4] = {0.f, 1.f, 0.f, 1.f};
+  float calib[4] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(), 0.f, 0.f};
+  int adam_steps = 0;
+  // training samples
+  DevBuf<uint16_t> X;
+  DevBuf<float> T;
+  DevBuf<uint16_t> V;
+  size_t count = 0;
+};
+
+namespace tess {
+
+tess_status alloc_all(tess_diagram* d);
+// bounds -> counting sort -> gather, enqueued only (graph-capturable)
+tess_status bin_enqueue(tess_diagram* d);
+// bin_enqueue + grid download + the check that every point was binned
+tess_status bin(tess_diagram* d);
+int pick_vpl(const tess_diagram* d);
+CellArgs base_args(tess_diagram* d, const Table* t);
+tess_status create_common(tess_ctx* ctx, size_t n, const double lo[3], const double hi[3],
+                          const tess_opts* opts, tess_diagram** out);
+// Cell kernel over grid positions work[0..nwork) (all cells when work is
+// null); capacity overflows are appended to the diagram's redo list when
+// push_redo is set.
+tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, int nwork, bool push_redo);
+// Work lists of grid positions per vertex capacity (slot 0: 64, 1: 128,
+// 2: 256); a null list with a count means every cell.
+struct Schedule {
+  const uint32_t* list[3] = {nullptr, nullptr, nullptr};
+  int count[3] = {0, 0, 0};
+};
+// Fills a schedule after binning (learned mode).
+using Planner = std::function<tess_status(tess_diagram*, Schedule&)>;
+// Binning, shell table, cell passes (with the capacity rerun), face offsets
+// and compaction; fills the diagram's timings.  plan == nullptr: plain.
+tess_status compute_cells(tess_diagram* d, const Planner* plan);
+
+}  // namespace tess

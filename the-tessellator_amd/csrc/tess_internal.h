@@ -39,6 +39,20 @@ struct Summary {
 // Faces staged per cell before CSR compaction.
 constexpr int kFaceStage = 64;
 
+// Learned search-radius model 27 -> 64 -> 64 -> 2.  Parameter vector layout
+// (FP32 master copy): W1 [32][64] (rows 27..31 are the zero feature pad),
+// W2 [64][64], W3 [64][16] (columns 2..15 unused), b1 [64], b2 [64],
+// b3 [16].  Weights row-major [input][output].
+constexpr int kMlpIn = 32;
+constexpr int kMlpFeatures = 27;
+constexpr int kMlpH = 64;
+constexpr int kMlpOut = 2;
+constexpr int kMlpOutPad = 16;
+constexpr int kMlpWeights = kMlpIn * kMlpH + kMlpH * kMlpH + kMlpH * kMlpOutPad;
+constexpr int kMlpBias = kMlpH + kMlpH + kMlpOutPad;
+constexpr int kMlpParams = kMlpWeights + kMlpBias;
+static_assert(kMlpParams == TESS_MLP_PARAMS, "parameter count");
+
 struct CellArgs {
   int n;                   // points in the set
   const double* spx;       // positions in grid order
@@ -72,6 +86,19 @@ struct CellArgs {
   double* exp_v;           // [256][3]
   int32_t* exp_ring;       // [256 faces][64] vertex ids, -1 terminated
   int32_t* exp_meta;       // nv, nf
+};
+
+// Persistent cell pass over many sets (one wavefront per cell at a time).
+struct BatchArgs {
+  const CellArgs* sets;   // per-set arguments (device array)
+  const uint64_t* items;  // (set << 32) | grid position
+  const uint32_t* order;  // nullable: item i is items[order[i]]
+  const uint32_t* count;  // number of items (device)
+  uint32_t* counter;      // next item to take; 0 before the launch
+  int chunk;              // items taken per atomic
+  uint64_t* redo;         // capacity overflows (nullable)
+  uint32_t* nredo;
+  int xcd_remap;          // 1: give each XCD a contiguous range of the list
 };
 
 }  // namespace tess

@@ -31,8 +31,15 @@ EXPORTS = [
     "tess_diagram_device_cells", "tess_diagram_copy_stats", "tess_diagram_timings", "tess_diagram_grid",
     "tess_diagram_copy_grid", "tess_cell_at_index", "tess_cell_destroy", "tess_cell_original_index",
     "tess_cell_volume", "tess_cell_neighbors", "tess_cell_face_count", "tess_cell_face",
-    "tess_cell_face_vertices", "tess_cell_vertices", "tess_diag_arith",
+    "tess_cell_face_vertices", "tess_cell_vertices", "tess_model_create", "tess_model_destroy",
+    "tess_model_get", "tess_model_set", "tess_model_save", "tess_model_load", "tess_model_add_samples",
+    "tess_model_sample_count", "tess_model_train", "tess_diagram_predict", "tess_diagram_compute_learned",
+    "tess_diagram_compute_scheduled", "tess_diagram_features", "tess_mlp_forward", "tess_mlp_gradient",
+    "tess_mfma_probe", "tess_batch_create", "tess_batch_destroy", "tess_batch_sets", "tess_batch_set_config",
+    "tess_batch_get_config", "tess_batch_compute", "tess_batch_tune", "tess_batch_total_faces",
+    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith",
 ]
+MLP_IN, MLP_PARAMS = 32, 7312
 
 
 class TessError(RuntimeError):
@@ -60,7 +67,12 @@ assert STATS_DTYPE.itemsize == C.sizeof(CellStats)
 class Timings(C.Structure):
     _fields_ = [("bin_ms", C.c_double), ("cells_ms", C.c_double), ("compact_ms", C.c_double),
                 ("total_ms", C.c_double), ("table_ms", C.c_double), ("flops", C.c_uint64),
-                ("fallback_cells", C.c_uint64), ("capacity_reruns", C.c_uint64)]
+                ("fallback_cells", C.c_uint64), ("capacity_reruns", C.c_uint64), ("plan_ms", C.c_double)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("first_capacity", C.c_int32), ("xcd_contiguous", C.c_int32), ("redo_chunk", C.c_int32),
+                ("learned", C.c_int32)]
 
 
 _lib = None
@@ -105,6 +117,32 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_cell_face": (C.c_int, [vp, sz, dp, dp]),
         "tess_cell_face_vertices": (sz, [vp, sz, dp, sz]),
         "tess_cell_vertices": (sz, [vp, dp, sz]),
+        "tess_model_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+        "tess_model_destroy": (None, [vp]),
+        "tess_model_get": (C.c_int, [vp, dp, dp, dp]),
+        "tess_model_set": (C.c_int, [vp, dp, dp, dp]),
+        "tess_model_save": (C.c_int, [vp, C.c_char_p]),
+        "tess_model_load": (C.c_int, [vp, C.c_char_p]),
+        "tess_model_add_samples": (C.c_int, [vp, vp]),
+        "tess_model_sample_count": (sz, [vp]),
+        "tess_model_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, dp]),
+        "tess_diagram_predict": (C.c_int, [vp, vp, dp]),
+        "tess_diagram_compute_learned": (C.c_int, [vp, vp]),
+        "tess_diagram_compute_scheduled": (C.c_int, [vp, dp, dp]),
+        "tess_diagram_features": (C.c_int, [vp, dp]),
+        "tess_mlp_forward": (C.c_int, [vp, dp, sz, dp]),
+        "tess_mlp_gradient": (C.c_int, [vp, dp, dp, sz, dp, dp]),
+        "tess_mfma_probe": (C.c_int, [vp, dp, dp, dp]),
+        "tess_batch_create": (C.c_int, [vp, sz, dp, dp, dp, dp, vp, C.POINTER(vp)]),
+        "tess_batch_destroy": (None, [vp]),
+        "tess_batch_sets": (sz, [vp]),
+        "tess_batch_set_config": (C.c_int, [vp, C.POINTER(BatchConfig)]),
+        "tess_batch_get_config": (C.c_int, [vp, C.POINTER(BatchConfig)]),
+        "tess_batch_compute": (C.c_int, [vp, vp]),
+        "tess_batch_tune": (C.c_int, [vp, vp, C.c_char_p, C.POINTER(BatchConfig)]),
+        "tess_batch_total_faces": (C.c_int, [vp, sz, C.POINTER(C.c_uint64)]),
+        "tess_batch_copy_cells": (C.c_int, [vp, sz, dp, dp, dp, dp, dp]),
+        "tess_batch_timings": (C.c_int, [vp, C.POINTER(Timings)]),
         "tess_diag_arith": (C.c_int, [vp, dp, dp, sz, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -145,6 +183,15 @@ class Context:
         s, r, q = (np.empty_like(a) for _ in range(3))
         self.check(self.lib.tess_diag_arith(self.h, _ptr(a), _ptr(b), a.size, _ptr(s), _ptr(r), _ptr(q)))
         return s, r, q
+
+    def mfma_probe(self, a_bits, b_bits):
+        """One 16x16x32 bf16 MFMA: a [16][32], b [32][16] bf16 bit patterns -> fp32 [16][16]."""
+        a = np.ascontiguousarray(a_bits, np.uint16)
+        b = np.ascontiguousarray(b_bits, np.uint16)
+        assert a.shape == (16, 32) and b.shape == (32, 16)
+        d = np.empty((16, 16), np.float32)
+        self.check(self.lib.tess_mfma_probe(self.h, _ptr(a), _ptr(b), _ptr(d)))
+        return d
 
     def close(self):
         if self.h:
@@ -284,6 +331,29 @@ class Diagram:
         self.ctx.check(self.lib.tess_diagram_copy_grid(self.h, _ptr(cell), _ptr(srt), _ptr(dl)))
         return cell, srt, dl
 
+    def compute_learned(self, model):
+        """All cells with the learned schedule (same results as compute)."""
+        self.ctx.check(self.lib.tess_diagram_compute_learned(self.h, model.h))
+
+    def compute_scheduled(self, log1p_c, t1=math.inf, t2=math.inf):
+        """All cells scheduled from given per-point log(1 + candidates)."""
+        v = np.ascontiguousarray(log1p_c, np.float32)
+        assert v.shape == (self.n,)
+        cal = np.array([t1, t2], np.float32)
+        self.ctx.check(self.lib.tess_diagram_compute_scheduled(self.h, _ptr(v), _ptr(cal)))
+
+    def predict(self, model):
+        """[n][2]: predicted final security radius and classified-candidate count."""
+        out = np.empty((self.n, 2), np.float32)
+        self.ctx.check(self.lib.tess_diagram_predict(self.h, model.h, _ptr(out)))
+        return out
+
+    def features(self):
+        """bf16 bit patterns [n][32] of the 3x3x3 occupancy features."""
+        out = np.empty((self.n, MLP_IN), np.uint16)
+        self.ctx.check(self.lib.tess_diagram_features(self.h, _ptr(out)))
+        return out
+
     def get_cell_at_index(self, index, search_radius=None, target_group=None):
         """interface.rs:186-208 + Cell::compute_voronoi_cell."""
         h = C.c_void_p()
@@ -291,6 +361,142 @@ class Diagram:
         g = -1 if target_group is None else int(target_group)
         self.ctx.check(self.lib.tess_cell_at_index(self.h, index, r, g, C.byref(h)))
         return Cell(self.lib, h)
+
+
+class Model:
+    """The learned search-radius model (tess_model): 27 -> 64 -> 64 -> 2, bf16 MFMA."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        ctx.check(self.lib.tess_model_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+        ctx._diagrams.add(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.lib.tess_model_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self):
+        p, norm, cal = np.empty(MLP_PARAMS, np.float32), np.empty(4, np.float32), np.empty(4, np.float32)
+        self.ctx.check(self.lib.tess_model_get(self.h, _ptr(p), _ptr(norm), _ptr(cal)))
+        return p, norm, cal
+
+    def set(self, params=None, norm=None, calib=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        cl = None if calib is None else np.ascontiguousarray(calib, np.float32)
+        if p is not None:
+            assert p.shape == (MLP_PARAMS,)
+        self.ctx.check(self.lib.tess_model_set(self.h, _ptr(p), _ptr(nm), _ptr(cl)))
+
+    def save(self, path):
+        self.ctx.check(self.lib.tess_model_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        self.ctx.check(self.lib.tess_model_load(self.h, os.fsencode(path)))
+
+    def add_samples(self, diagram):
+        self.ctx.check(self.lib.tess_model_add_samples(self.h, diagram.h))
+
+    def sample_count(self):
+        return int(self.lib.tess_model_sample_count(self.h))
+
+    def train(self, epochs=4, batch=4096, lr=1e-3, seed=0):
+        loss = np.empty(max(epochs, 1), np.float32)
+        self.ctx.check(self.lib.tess_model_train(self.h, epochs, batch, lr, seed, _ptr(loss)))
+        return loss[:epochs]
+
+    def forward(self, x_bits):
+        x = np.ascontiguousarray(x_bits, np.uint16)
+        y = np.empty((x.shape[0], 2), np.float32)
+        self.ctx.check(self.lib.tess_mlp_forward(self.h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def gradient(self, x_bits, t):
+        x = np.ascontiguousarray(x_bits, np.uint16)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.lib.tess_mlp_gradient(self.h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+
+class Batch:
+    """Many independent point sets tessellated together (tess_batch).
+
+    sets: list of (xyz [n][3], lo [3], hi [3]).  Per set, results are bitwise
+    those of a Diagram of the same points and box."""
+
+    def __init__(self, ctx: Context, sets, density=1.25, collect_stats=False):
+        self.ctx, self.lib = ctx, ctx.lib
+        self._pts = [np.ascontiguousarray(p, np.float64).reshape(-1, 3) for p, _, _ in sets]
+        self.sizes = [len(p) for p in self._pts]
+        ptrs = (C.c_void_p * len(sets))(*[p.ctypes.data for p in self._pts])
+        n = np.array(self.sizes, np.uint64)
+        lo = np.ascontiguousarray([l for _, l, _ in sets], np.float64)
+        hi = np.ascontiguousarray([h for _, _, h in sets], np.float64)
+        o = Opts()
+        self.lib.tess_opts_default(C.byref(o))
+        o.cell_density, o.collect_stats = density, int(collect_stats)
+        h = C.c_void_p()
+        ctx.check(self.lib.tess_batch_create(ctx.h, len(sets), C.cast(ptrs, C.c_void_p), _ptr(n), _ptr(lo), _ptr(hi),
+                                             C.byref(o), C.byref(h)))
+        self.h = h
+        self._pts = None
+        ctx._diagrams.add(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.lib.tess_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self):
+        k = BatchConfig()
+        self.ctx.check(self.lib.tess_batch_get_config(self.h, C.byref(k)))
+        return {f: getattr(k, f) for f, _ in BatchConfig._fields_}
+
+    def set_config(self, **kw):
+        cur = self.config()
+        cur.update(kw)
+        k = BatchConfig(**cur)
+        self.ctx.check(self.lib.tess_batch_set_config(self.h, C.byref(k)))
+
+    def compute(self, model=None):
+        self.ctx.check(self.lib.tess_batch_compute(self.h, model.h if model is not None else None))
+
+    def tune(self, model=None, cache_path=None):
+        k = BatchConfig()
+        self.ctx.check(self.lib.tess_batch_tune(self.h, model.h if model is not None else None,
+                                                os.fsencode(cache_path) if cache_path else None, C.byref(k)))
+        return {f: getattr(k, f) for f, _ in BatchConfig._fields_}
+
+    def cells(self, s):
+        tf = C.c_uint64()
+        self.ctx.check(self.lib.tess_batch_total_faces(self.h, s, C.byref(tf)))
+        n, f = self.sizes[s], tf.value
+        vol, fc = np.empty(n), np.empty(n, np.uint32)
+        fo, nb, ar = np.empty(n + 1, np.uint64), np.empty(f, np.int32), np.empty(f)
+        self.ctx.check(self.lib.tess_batch_copy_cells(self.h, s, _ptr(vol), _ptr(fc), _ptr(fo), _ptr(nb), _ptr(ar)))
+        return dict(volume=vol, face_count=fc, face_offset=fo, neighbor=nb, area=ar)
+
+    def timings(self):
+        t = Timings()
+        self.ctx.check(self.lib.tess_batch_timings(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timings._fields_}
 
 
 # ------------------------------------------------------------ workloads
