@@ -117,7 +117,12 @@ def bench_clustered(ctx, w, args):
     n = len(pts)
     lo, hi = np.zeros(3), np.ones(3)
     path = os.path.join(tempfile.gettempdir(), "tess_clustered4m_model.bin")
-    m, train = train_model(ctx, w, path)
+    if args.model:
+        m = tess.Model(ctx)
+        m.load(args.model)
+        train = {"loaded": args.model}
+    else:
+        m, train = train_model(ctx, w, path)
     # plain kernel on the same points
     d = tess.Diagram(ctx, pts, lo, hi, density=w["density"])
 
@@ -212,6 +217,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--vcap", type=int, default=0, help="vertex capacity of the plain kernel (0: automatic)")
+    ap.add_argument("--model", default="", help="clustered4m: load this trained model instead of training one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

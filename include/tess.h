@@ -79,7 +79,8 @@ enum {
   TESS_CELL_CAPACITY = 1u,   /* cell needed a larger kernel variant */
   TESS_CELL_TABLE = 2u,      /* shell table exhausted before the stop */
   TESS_CELL_DEGENERATE = 4u, /* topology check failed */
-  TESS_CELL_FALLBACK = 8u    /* learned mode: prediction under-sized the batch */
+  TESS_CELL_FALLBACK = 8u,   /* learned mode: prediction under-sized the batch */
+  TESS_CELL_RERUN = 16u      /* recomputed at the largest capacity after overflowing */
 };
 
 /* Device pointers to a computed diagram's results (valid until destroy). */
@@ -191,7 +192,9 @@ size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
  * W3[64][16] (row-major [in][out]; columns 2..15 of W3 unused), b1[64],
  * b2[64], b3[16].  norm[4] = mean0, std0, mean1, std1 of the targets.
  * calib[4] = routing thresholds on log(1 + C): cells above calib[0] start
- * at vertex capacity 128, above calib[1] at 256; calib[2..3] reserved. */
+ * at vertex capacity 128, above calib[1] at 256; calib[2..3] reserved.
+ * Training sets them to the predicted value above which at least one cell
+ * in five of the training samples outgrew 64 (resp. 128) vertices. */
 #define TESS_MLP_IN 32
 #define TESS_MLP_PARAMS 7312
 typedef struct tess_model tess_model;

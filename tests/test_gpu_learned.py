@@ -168,12 +168,16 @@ def test_fallback_cells_flagged(ctx):
     d.compute()
     plain = d.cells()
     reruns = d.timings()["capacity_reruns"]
-    over = d.stats()["max_vertices"] > 64
+    st0 = d.stats()
+    over = st0["max_vertices"] > 64
     assert over.sum() > 0
+    rerun = (st0["status"] & tess.CELL_RERUN) != 0  # plain mode flags its reruns too
+    assert int(rerun.sum()) == reruns and np.all(rerun[over])
+    assert not np.any(st0["status"] & tess.CELL_FALLBACK)
     d.compute_scheduled(np.zeros(len(pts), np.float32), t1=np.inf, t2=np.inf)
     _same(d.cells(), plain)
     flagged = (d.stats()["status"] & tess.CELL_FALLBACK) != 0
-    assert np.all(flagged[over])
+    assert np.all(flagged[over]) and np.array_equal(flagged, rerun)
     assert int(flagged.sum()) == d.timings()["fallback_cells"] == reruns
 
 

@@ -338,9 +338,9 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
         (st = c->hip(hipMemsetAsync(&d->summary.p->capacity, 0, sizeof(unsigned int), s), "memset")) != TESS_OK ||
         (st = run_cells(d, t, 4, d->redo_list.p, (int)nredo, false)) != TESS_OK)
       return st;
-    if (plan &&
-        (st = c->hip(launch_mark_status(d->redo_list.p, (int)nredo, d->sorted.p, d->status.p,
-                                        d->opts.collect_stats ? d->stats.p : nullptr, TESS_CELL_FALLBACK, s),
+    if ((st = c->hip(launch_mark_status(d->redo_list.p, (int)nredo, d->sorted.p, d->status.p,
+                                        d->opts.collect_stats ? d->stats.p : nullptr,
+                                        TESS_CELL_RERUN | (plan ? TESS_CELL_FALLBACK : 0u), s),
                      "mark")) != TESS_OK)
       return st;
   }

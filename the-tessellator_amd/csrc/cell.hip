@@ -387,7 +387,9 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
 #pragma unroll
     for (int r = 0; r < FPL; ++r) C.FA[r] = ballot(S.fmark[lane + 64 * r] != 0);
   } else {
-    C.FA[nf >> 6] |= 1ull << (nf & 63);
+#pragma unroll
+    for (int r = 0; r < FPL; ++r)
+      if ((nf >> 6) == r) C.FA[r] |= 1ull << (nf & 63);
   }
   wave_sync();
 #pragma unroll
@@ -798,6 +800,12 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         bool any = false;
 #pragma unroll
         for (int r = 0; r < VPL; ++r) {
+          if (r > 0 && C.VA[r] == 0) {  // vertex slots fill from the bottom
+            cv[r] = C_IN;
+            dv[r] = 0.0;
+            OM[r] = 0;
+            continue;
+          }
           const bool al = (C.VA[r] >> lane) & 1ull;
           uint8_t cc = C_IN;
           double d = 0.0;
@@ -1081,7 +1089,7 @@ __global__ __launch_bounds__(64) void k_cells_items(BatchArgs B, uint32_t n) {
 // (*B.count): every wavefront takes `chunk` items at a time from a shared
 // counter until the list is exhausted, so the loop ends for every wavefront.
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) void k_cells_queue(BatchArgs B) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_cells_queue(BatchArgs B) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
   const uint32_t n = *B.count;

@@ -349,7 +349,8 @@ __global__ void k_adam(const float* __restrict__ grad, int nslots, float* __rest
 
 // Training targets from the plain search's counters: log(R / s_min) with R
 // the final security radius, and log(1 + classified candidates); vmax: the
-// largest vertex count the cell reached.
+// largest vertex count the cell reached (at least 65 when the cell had to be
+// rerun at a larger capacity: its cuts ran out of the 64 slots).
 __global__ void k_targets(const tess_cell_stats* __restrict__ st, int n, double smin, float* __restrict__ T,
                           uint16_t* __restrict__ vmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -357,7 +358,9 @@ __global__ void k_targets(const tess_cell_stats* __restrict__ st, int n, double 
   const tess_cell_stats c = st[i];
   T[2 * i] = (float)log(sqrt(c.r2) / smin);
   T[2 * i + 1] = (float)log1p((double)c.classified);
-  vmax[i] = (uint16_t)(c.max_vertices < 65535u ? c.max_vertices : 65535u);
+  uint32_t v = c.max_vertices < 65535u ? c.max_vertices : 65535u;
+  if ((c.status & TESS_CELL_RERUN) && v < 65u) v = 65u;
+  vmax[i] = (uint16_t)v;
 }
 
 // Sum of nslots gradient slots in slot order.

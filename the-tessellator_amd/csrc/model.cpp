@@ -305,10 +305,11 @@ tess_status tess_model_train(tess_model* m, int epochs, int batch, float lr, uin
     if (loss_out) loss_out[ep] = (float)(tot / ((double)nbatches * batch));
   }
   // routing thresholds: predicted log(1 + C) of the samples against the
-  // largest vertex count they reached.  A cell goes to capacity 128 when its
-  // prediction exceeds t1, the 10th percentile of the predictions of cells
-  // that outgrew 64 vertices (and to 256 likewise for 128); when fewer than
-  // 32 samples outgrew a capacity, that route stays closed.
+  // largest vertex count they reached.  Predictions are binned (width 1/4);
+  // threshold r is the upper edge of the highest bin (with at least 20
+  // samples) in which fewer than one cell in five outgrew caps[r] vertices,
+  // so the cells routed to the larger capacity are those whose bins mostly
+  // need it.  No bin at or above one in five: the route stays closed.
   DevBuf<float> Y;
   if (Y.alloc(2 * S) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
   std::vector<float> y(2 * S);
@@ -318,16 +319,32 @@ tess_status tess_model_train(tess_model* m, int epochs, int batch, float lr, uin
       (st = c->hip(hipMemcpyAsync(V.data(), m->V.p, 2 * S, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "calibrate")) != TESS_OK)
     return st;
+  constexpr int kBins = 80;  // log(1 + C) in [0, 20)
   const int caps[2] = {64, 128};
   for (int r = 0; r < 2; ++r) {
-    std::vector<float> over;
-    for (size_t i = 0; i < S; ++i)
-      if (V[i] > caps[r]) over.push_back(y[2 * i + 1] * m->norm[3] + m->norm[2]);
+    std::vector<uint64_t> cnt(kBins, 0), over(kBins, 0);
+    bool any = false;
+    for (size_t i = 0; i < S; ++i) {
+      const float v = y[2 * i + 1] * m->norm[3] + m->norm[2];
+      const int bin = std::min(kBins - 1, std::max(0, (int)std::floor(v * 4.0f)));
+      cnt[bin]++;
+      if (V[i] > caps[r]) {
+        over[bin]++;
+        any = true;
+      }
+    }
     float t = std::numeric_limits<float>::infinity();
-    if (over.size() >= 32) {
-      const size_t q = over.size() / 10;
-      std::nth_element(over.begin(), over.begin() + q, over.end());
-      t = over[q];
+    if (any) {
+      int hi = -1;  // highest populated bin below one in five
+      bool high_seen = false;
+      for (int bin = 0; bin < kBins; ++bin) {
+        if (cnt[bin] < 20) continue;
+        if (5 * over[bin] < cnt[bin])
+          hi = bin;
+        else
+          high_seen = true;
+      }
+      if (high_seen) t = (hi + 1) / 4.0f;
     }
     m->calib[r] = t;
   }

@@ -22,7 +22,7 @@ TESS_OK = 0
 STATUS = {0: "TESS_OK", 1: "TESS_E_INVALID", 2: "TESS_E_NOMEM", 3: "TESS_E_DEVICE",
           4: "TESS_E_CAPACITY", 5: "TESS_E_DEGENERATE", 6: "TESS_E_STATE", 7: "TESS_E_NOTFOUND"}
 MODE_PLAIN, MODE_LEARNED = 0, 1
-CELL_CAPACITY, CELL_TABLE, CELL_DEGENERATE, CELL_FALLBACK = 1, 2, 4, 8
+CELL_CAPACITY, CELL_TABLE, CELL_DEGENERATE, CELL_FALLBACK, CELL_RERUN = 1, 2, 4, 8, 16
 
 EXPORTS = [
     "tess_opts_default", "tess_ctx_create", "tess_ctx_destroy", "tess_last_error", "tess_ctx_stream",
