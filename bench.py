@@ -76,6 +76,12 @@ def cpu_baseline(pts, lo, hi, density, sample, threads):
     return len(cells) / r["seconds"], r["seconds"], len(cells)
 
 
+def fixed_config(text):
+    """'64,0,4,1' -> first_capacity, xcd_contiguous, redo_chunk, learned."""
+    v = [int(x) for x in text.split(",")]
+    return dict(zip(("first_capacity", "xcd_contiguous", "redo_chunk", "learned"), v))
+
+
 def timed(fn, steps, warmup, ctx):
     for _ in range(warmup):
         fn()
@@ -136,8 +142,26 @@ def bench_clustered(ctx, w, args):
     # learned-radius batched path
     b = tess.Batch(ctx, [(pts, lo, hi)], density=w["density"])
     cache = os.path.join(tempfile.gettempdir(), "tess_tune_cache.txt")
-    cfg = b.tune(m, cache_path=cache)
-    log(f"tuned config: {cfg}")
+    if args.batch_config:
+        b.set_config(**fixed_config(args.batch_config))
+        cfg = b.config()
+    else:
+        cfg = b.tune(m, cache_path=cache)
+    log(f"config: {cfg}")
+    sweep = []
+    for k in ((dict(first_capacity=64, xcd_contiguous=0, learned=0), dict(first_capacity=64, xcd_contiguous=1, learned=0),
+               dict(first_capacity=128, xcd_contiguous=0, learned=0), dict(first_capacity=64, xcd_contiguous=0, learned=1))
+              if args.sweep else ()):
+        b.set_config(**k)
+        b.compute(m)
+        t = []
+        for _ in range(2):
+            b.compute(m)
+            t.append(b.timings())
+        sweep.append(dict(k, total_ms=mean(t, "total_ms"), cells_ms=mean(t, "cells_ms"), plan_ms=mean(t, "plan_ms"),
+                          reruns=int(t[-1]["capacity_reruns"])))
+        log(f"sweep {sweep[-1]}")
+    b.set_config(**cfg)
 
     def step():
         b.compute(m)
@@ -156,6 +180,7 @@ def bench_clustered(ctx, w, args):
         "fallback_cells": int(tims[-1]["fallback_cells"]),
         "fallback_rate": tims[-1]["fallback_cells"] / n,
         "tuned_config": cfg,
+        "config_sweep": sweep,
         "model": train,
     }
     return pts, lo, hi, n, ms, tims, got, extra, {"mode": "learned-batched"}
@@ -184,7 +209,11 @@ def bench_batch(ctx, w, args):
     log(f"batch: {len(sets)} sets, {n} points")
     b = tess.Batch(ctx, sets, density=w["density"])
     cache = os.path.join(tempfile.gettempdir(), "tess_tune_cache.txt")
-    cfg = b.tune(None, cache_path=cache)
+    if args.batch_config:
+        b.set_config(**fixed_config(args.batch_config))
+        cfg = b.config()
+    else:
+        cfg = b.tune(None, cache_path=cache)
 
     def step():
         b.compute()
@@ -218,6 +247,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--vcap", type=int, default=0, help="vertex capacity of the plain kernel (0: automatic)")
     ap.add_argument("--model", default="", help="clustered4m: load this trained model instead of training one")
+    ap.add_argument("--batch-config", default="", help="skip the autotuner: first_capacity,xcd_contiguous,redo_chunk,"
+                                                       "learned (e.g. 64,0,4,0)")
+    ap.add_argument("--sweep", action="store_true", help="clustered4m: also time a fixed set of batch configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -263,6 +295,14 @@ def main():
             "traffic": None,
         },
     }
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        # HBM bytes per launch of the dominant cell kernel, from the rocprofv3
+        # FETCH_SIZE / WRITE_SIZE runs of this workload, copy-kernel scaled
+        rec = json.load(open(tpath)).get(args.workload)
+        if rec:
+            line["roofline"]["traffic"] = rec["bytes_per_launch"]
+            line["roofline"]["traffic_kernel"] = rec["kernel"]
     if faces is not None:
         # algorithmic HBM bytes of the cell passes: site position + id (28 B),
         # volume + face count + status (16 B), 12 B per staged face

@@ -122,3 +122,28 @@ def test_batch_rejects_bad_input(ctx):
         b.set_config(first_capacity=100)
     with pytest.raises(tess.TessError):
         b.cells(0)  # before compute
+
+
+def test_fcc_lattices_learned_batched(ctx):
+    """Near-degenerate input (jittered FCC, many near-co-spherical vertices),
+    learned order and batch together: bitwise equal to single diagrams."""
+    sets = []
+    for i in range(3):
+        pts = tess.fcc(4000 + i, ncell=16, a=1.0, sigma=1e-3)
+        sets.append((pts, np.zeros(3), np.full(3, 16.0)))
+    ref = []
+    for pts, lo, hi in sets:
+        d = tess.Diagram(ctx, pts, lo, hi, density=5.0)
+        d.compute()
+        ref.append(d.cells())
+    m = tess.Model(ctx, seed=3)
+    d = tess.Diagram(ctx, sets[0][0], sets[0][1], sets[0][2], density=5.0, collect_stats=True)
+    d.compute()
+    m.add_samples(d)
+    m.train(epochs=2, batch=2048, lr=2e-3, seed=4)
+    b = tess.Batch(ctx, sets, density=5.0)
+    b.set_config(learned=1)
+    b.compute(m)
+    for s in range(len(sets)):
+        _same(b.cells(s), ref[s])
+    assert b.timings()["fallback_cells"] <= b.timings()["capacity_reruns"]

@@ -145,6 +145,9 @@ __device__ __forceinline__ int axis_index(double v, double vmin, double vmax, do
 
 __device__ __forceinline__ double stop_thr(double r2) { return (4.0 * r2) * (1.0 + 1e-9); }
 
+// A chunk with at least this many staged candidates is pre-tested lane-parallel.
+constexpr int kPreMin = 16;
+
 __device__ __forceinline__ bool lex_less(double ax, double ay, double az, double bx, double by, double bz) {
   if (ax != bx) return ax < bx;
   if (ay != by) return ay < by;
@@ -784,14 +787,45 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       }
       const int nkeep = popc(km);
       wave_sync();
-      for (int q = 0; q < nkeep; ++q) {
-        const int ec = uni_i(S.u.cand.ent[q]);
-        if (ec >= e_stop) break;
-        const double cd2 = uni_d(S.u.cand.d2[q]);
-        if (cd2 > stop_thr(r2)) continue;
+      const uint64_t all_kept = nkeep >= 64 ? ~0ull : ((1ull << nkeep) - 1ull);
+      // Pre-test, lane = candidate: which staged candidates cut the cell as it
+      // is now.  One that does not can never cut (the cell only shrinks), so
+      // only the others need the sequential test; the ones skipped are still
+      // counted as classified at their turn, exactly as the sequential pass
+      // would count them.  Same arithmetic as the sequential test.
+      uint64_t mc = all_kept;
+      if (nkeep >= kPreMin) {
+        bool cut = false;
+        if (lane < nkeep) {
+          const double nx = S.u.cand.n[lane][0], ny = S.u.cand.n[lane][1];
+          const double nz = S.u.cand.n[lane][2], off = S.u.cand.n[lane][3];
+#pragma unroll
+          for (int r = 0; r < VPL; ++r) {
+            uint64_t m = C.VA[r];
+            while (m) {
+              const int v = 64 * r + ctz64(m);
+              m &= m - 1ull;
+              const double d = (nx * S.vx[v] + ny * S.vy[v]) + nz * S.vz[v];
+              cut = cut || (d - off > kTol);
+            }
+          }
+        }
+        mc = ballot(cut);
+      }
+      uint64_t pv = all_kept;  // pending and still within the stop entry / radius
+      int nv = popc_multi(C.VA);
+      for (;;) {
+        const uint64_t next = pv & mc;
+        const uint64_t before = next ? (pv & ((1ull << ctz64(next)) - 1ull)) : pv;
+        const int nb = popc(before);
+        classified += (uint32_t)nb;
+        flops += (uint64_t)nb * (13ull + 6ull * (uint64_t)nv);
+        pv &= ~before;
+        if (!next) break;
+        const int q = ctz64(next);
+        pv &= pv - 1ull;
         const double cnx = uni_d(S.u.cand.n[q][0]), cny = uni_d(S.u.cand.n[q][1]);
         const double cnz = uni_d(S.u.cand.n[q][2]), coff = uni_d(S.u.cand.n[q][3]);
-        int nv = popc_multi(C.VA);
         classified++;
         flops += 13 + 6ull * (uint64_t)nv;
         uint64_t OM[VPL];
@@ -829,6 +863,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         wave_sync();
         int newv = 0;
         const int32_t ctag = uni_i(S.u.cand.tag[q]);
+        const int ec = uni_i(S.u.cand.ent[q]);
         int rc = clip_fast(S, C, OM, coff, ctag, lane, &newv);
         if (rc == CUT_GENERAL) rc = clip(S, C, OM, coff, ctag, lane, &newv);
         if (rc != CUT_OK) {
@@ -842,12 +877,14 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         flops += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
         r2 = cell_r2(C, lane);
         e_lastcut = ec;
-        const uint64_t sm2 =
-            ballot(lane < lim && ebase + lane > e_lastcut && (key > stop_thr(r2) || key > rlim));
+        const double thr = stop_thr(r2);
+        const uint64_t sm2 = ballot(lane < lim && ebase + lane > e_lastcut && (key > thr || key > rlim));
         if (sm2) {
           e_stop = ebase + ctz64(sm2);
           stop_here = true;
         }
+        // the smaller cell reaches fewer of the pending candidates
+        pv &= ballot(lane < nkeep && S.u.cand.ent[lane] < e_stop && !(S.u.cand.d2[lane] > thr));
       }
       streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
       if (stop_here && base + 64 < total) {
@@ -1038,7 +1075,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 
 // One cell per wavefront over a set's grid positions (all, or a work list).
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) void k_cells(CellArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells(CellArgs A) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
   int w = blockIdx.x;
@@ -1074,7 +1111,7 @@ __device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<
 // the hardware dispatcher hands out the items in list order as wavefront
 // slots free up, which is the work queue of the pass.
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) void k_cells_items(BatchArgs B, uint32_t n) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells_items(BatchArgs B, uint32_t n) {
   __shared__ Lds<VPL, PPL, FPL> S;
   uint32_t i = blockIdx.x;
   if (B.xcd_remap) {
