@@ -77,9 +77,9 @@ def cpu_baseline(pts, lo, hi, density, sample, threads):
 
 
 def fixed_config(text):
-    """'64,0,4,1' -> first_capacity, xcd_contiguous, redo_chunk, learned."""
+    """'64,0,4,1' -> first_capacity, xcd_chunk, redo_chunk, learned."""
     v = [int(x) for x in text.split(",")]
-    return dict(zip(("first_capacity", "xcd_contiguous", "redo_chunk", "learned"), v))
+    return dict(zip(("first_capacity", "xcd_chunk", "redo_chunk", "learned"), v))
 
 
 def timed(fn, steps, warmup, ctx):
@@ -130,7 +130,7 @@ def bench_clustered(ctx, w, args):
     else:
         m, train = train_model(ctx, w, path)
     # plain kernel on the same points
-    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"])
+    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"], xcd_chunk=args.xcd_chunk)
 
     def plain_step():
         d.compute()
@@ -149,8 +149,9 @@ def bench_clustered(ctx, w, args):
         cfg = b.tune(m, cache_path=cache)
     log(f"config: {cfg}")
     sweep = []
-    for k in ((dict(first_capacity=64, xcd_contiguous=0, learned=0), dict(first_capacity=64, xcd_contiguous=1, learned=0),
-               dict(first_capacity=128, xcd_contiguous=0, learned=0), dict(first_capacity=64, xcd_contiguous=0, learned=1))
+    for k in ((dict(first_capacity=64, xcd_chunk=0, learned=0), dict(first_capacity=64, xcd_chunk=256, learned=0),
+               dict(first_capacity=64, xcd_chunk=4096, learned=0), dict(first_capacity=128, xcd_chunk=0, learned=0),
+               dict(first_capacity=64, xcd_chunk=0, learned=1))
               if args.sweep else ()):
         b.set_config(**k)
         b.compute(m)
@@ -190,7 +191,7 @@ def bench_uniform(ctx, w, args):
     pts = make_points(w)
     n = len(pts)
     lo, hi = np.zeros(3), np.ones(3)
-    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"], vertex_capacity=args.vcap)
+    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"], vertex_capacity=args.vcap, xcd_chunk=args.xcd_chunk)
 
     def step():
         d.compute()
@@ -246,8 +247,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--vcap", type=int, default=0, help="vertex capacity of the plain kernel (0: automatic)")
+    ap.add_argument("--xcd-chunk", type=int, default=0,
+                    help="plain kernel: cells each XCD takes in a row (0: automatic)")
     ap.add_argument("--model", default="", help="clustered4m: load this trained model instead of training one")
-    ap.add_argument("--batch-config", default="", help="skip the autotuner: first_capacity,xcd_contiguous,redo_chunk,"
+    ap.add_argument("--batch-config", default="", help="skip the autotuner: first_capacity,xcd_chunk,redo_chunk,"
                                                        "learned (e.g. 64,0,4,0)")
     ap.add_argument("--sweep", action="store_true", help="clustered4m: also time a fixed set of batch configurations")
     args = ap.parse_args()

@@ -60,7 +60,8 @@ typedef struct {
   int32_t mode;            /* TESS_MODE_* */
   int32_t vertex_capacity; /* 0: automatic; else 64, 128 or 256 */
   int32_t collect_stats;   /* 1: keep per-cell search counters */
-  int32_t reserved;
+  int32_t xcd_chunk;       /* whole-set passes: cells each XCD takes in a row
+                              (0: automatic, 256) */
 } tess_opts;
 
 /* Per-cell search counters (collect_stats = 1). */
@@ -244,7 +245,8 @@ tess_status tess_mfma_probe(tess_ctx* ctx, const uint16_t* a, const uint16_t* b,
 typedef struct tess_batch tess_batch;
 typedef struct {
   int32_t first_capacity; /* vertex capacity of the first pass: 64, 128, 256 */
-  int32_t xcd_contiguous; /* 1: each XCD takes a contiguous range of the list */
+  int32_t xcd_chunk;      /* 0: list entries dealt to the XCDs one by one;
+                             K > 0: in runs of K consecutive entries */
   int32_t redo_chunk;     /* cells a wavefront takes per grab in the rerun pass */
   int32_t learned;        /* 1: heaviest predicted cells first (needs a model) */
 } tess_batch_config;
@@ -258,7 +260,9 @@ tess_status tess_batch_set_config(tess_batch* b, const tess_batch_config* cfg);
 tess_status tess_batch_get_config(const tess_batch* b, tess_batch_config* cfg);
 /* All cells of all sets; m may be NULL unless config.learned = 1. */
 tess_status tess_batch_compute(tess_batch* b, const tess_model* m);
-/* Time the candidate configurations on this batch and keep the fastest.
+/* Time the candidate configurations on this batch and keep the fastest:
+ * first_capacity 64 or 128 x xcd_chunk 0, 512 or 4096, and with a model the
+ * learned order at capacity 64 with xcd_chunk 0 or 512 (redo_chunk 4).
  * cache_path (nullable): a text file of earlier results keyed by device and
  * batch shape; a hit skips the timing, a miss appends a line. */
 tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cache_path, tess_batch_config* best);

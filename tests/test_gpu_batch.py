@@ -55,7 +55,7 @@ def test_batch_matches_single_sets(ctx, uneven):
     assert t["total_ms"] > 0 and t["cells_ms"] > 0
 
 
-@pytest.mark.parametrize("cfg", [dict(first_capacity=128), dict(first_capacity=256), dict(xcd_contiguous=1),
+@pytest.mark.parametrize("cfg", [dict(first_capacity=128), dict(first_capacity=256), dict(xcd_chunk=256), dict(xcd_chunk=7),
                                  dict(redo_chunk=1), dict(redo_chunk=16)])
 def test_batch_configs_bitwise(ctx, uneven, cfg):
     sets, ref = uneven

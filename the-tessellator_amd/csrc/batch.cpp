@@ -55,7 +55,7 @@ struct Graph {
 struct tess_batch {
   tess_ctx* ctx = nullptr;
   tess_opts opts;
-  tess_batch_config cfg{64, 0, 4, 0};
+  tess_batch_config cfg{64, 0, 4, 0};  // capacity 64, interleaved, chunk 4, list order
   size_t nsets = 0, N = 0;
   std::vector<size_t> n, pbase;                        // points per set, first point
   std::vector<size_t> bbase, cbase, sbase, partbase;  // delim, bins, scan sums, partial bounds
@@ -304,14 +304,14 @@ tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32
       if (!cnt) continue;
       a.order = list_base + bounds[slot];
       a.redo = slot < 2 ? b->redo.p : nullptr;
-      a.xcd_remap = 0;
+      a.xcd_chunk = 0;
       if ((st = c->hip(launch_cells_items(kVpl[slot], a, cnt, s), "cell pass")) != TESS_OK) return st;
     }
   } else {
     const int vpl = vpl_of(b->cfg.first_capacity);
     a.order = nullptr;
     a.redo = vpl < 4 ? b->redo.p : nullptr;
-    a.xcd_remap = b->cfg.xcd_contiguous;
+    a.xcd_chunk = b->cfg.xcd_chunk;
     if ((st = c->hip(launch_cells_items(vpl, a, (uint32_t)b->N, s), "cell pass")) != TESS_OK) return st;
   }
   // rerun at the largest capacity: persistent over the device-counted list
@@ -361,7 +361,7 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
   uint32_t bounds[4] = {0, (uint32_t)b->N, (uint32_t)b->N, (uint32_t)b->N};
   if (learned && (st = learned_order(b, m, bounds)) != TESS_OK) return st;
   (void)hipEventRecord(ev[2], s);
-  std::vector<int64_t> key = {b->cfg.first_capacity, b->cfg.xcd_contiguous, b->cfg.redo_chunk, b->cfg.learned,
+  std::vector<int64_t> key = {b->cfg.first_capacity, b->cfg.xcd_chunk, b->cfg.redo_chunk, b->cfg.learned,
                               bounds[1], bounds[2]};
   if (!b->gcells.exec || key != b->gcells_key) {
     if ((st = capture_begin(c)) != TESS_OK) return st;
@@ -424,7 +424,7 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
 
 bool valid_config(const tess_batch_config& k) {
   return (k.first_capacity == 64 || k.first_capacity == 128 || k.first_capacity == 256) &&
-         (k.xcd_contiguous == 0 || k.xcd_contiguous == 1) && k.redo_chunk >= 1 && k.redo_chunk <= 64 &&
+         k.xcd_chunk >= 0 && k.xcd_chunk <= (1 << 24) && k.redo_chunk >= 1 && k.redo_chunk <= 64 &&
          (k.learned == 0 || k.learned == 1);
 }
 
@@ -580,7 +580,7 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
   tess_ctx* c = b->ctx;
   const bool can_learn = m && m->wbf.p;
   const std::string sig = signature(b, can_learn);
-  // cache lookup: "signature first_capacity xcd_contiguous redo_chunk learned ms"
+  // cache lookup: "signature first_capacity xcd_chunk redo_chunk learned ms"
   if (cache_path) {
     std::ifstream in(cache_path);
     std::string line;
@@ -589,7 +589,7 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
       std::string s;
       tess_batch_config k;
       double ms;
-      if (is >> s >> k.first_capacity >> k.xcd_contiguous >> k.redo_chunk >> k.learned >> ms && s == sig &&
+      if (is >> s >> k.first_capacity >> k.xcd_chunk >> k.redo_chunk >> k.learned >> ms && s == sig &&
           valid_config(k) && (!k.learned || can_learn)) {
         b->cfg = k;
         if (best) *best = k;
@@ -599,8 +599,9 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
   }
   std::vector<tess_batch_config> cand;
   for (int cap : {64, 128})
-    for (int xcd : {0, 1}) cand.push_back({cap, xcd, 4, 0});
-  if (can_learn) cand.push_back({64, 0, 4, 1});
+    for (int xcd : {0, 512, 4096}) cand.push_back({cap, xcd, 4, 0});
+  if (can_learn)
+    for (int xcd : {0, 512}) cand.push_back({64, xcd, 4, 1});
   double best_ms = 1e300;
   tess_batch_config bk = b->cfg;
   for (const tess_batch_config& k : cand) {
@@ -622,7 +623,7 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
   if (best) *best = bk;
   if (cache_path) {
     std::ofstream out(cache_path, std::ios::app);
-    out << sig << ' ' << bk.first_capacity << ' ' << bk.xcd_contiguous << ' ' << bk.redo_chunk << ' ' << bk.learned
+    out << sig << ' ' << bk.first_capacity << ' ' << bk.xcd_chunk << ' ' << bk.redo_chunk << ' ' << bk.learned
         << ' ' << best_ms << '\n';
   }
   return TESS_OK;

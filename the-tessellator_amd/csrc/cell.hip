@@ -143,6 +143,20 @@ __device__ __forceinline__ int axis_index(double v, double vmin, double vmax, do
   return i < cpd - 1 ? i : cpd - 1;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b & 7).
+// With runs of K list entries dealt to the XCDs in turn, XCD x's j-th block
+// takes entry j % K of run (j / K) * 8 + x: neighbouring cells share an L2
+// while the heavy regions of the list still spread over all XCDs.
+__host__ __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t K) {
+  const uint32_t x = b & 7u, j = b >> 3;
+  return ((j / K) * 8u + x) * K + (j % K);
+}
+// Blocks needed to cover n entries (whole rounds of 8 runs).
+__host__ __device__ __forceinline__ uint32_t xcd_grid(uint32_t n, uint32_t K) {
+  const uint32_t round = 8u * K;
+  return (n + round - 1) / round * round;
+}
+
 __device__ __forceinline__ double stop_thr(double r2) { return (4.0 * r2) * (1.0 + 1e-9); }
 
 // A chunk with at least this many staged candidates is pre-tested lane-parallel.
@@ -1078,11 +1092,7 @@ template <int VPL, int PPL, int FPL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells(CellArgs A) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
-  int w = blockIdx.x;
-  if (A.xcd_remap) {
-    const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
-    w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
+  const int w = A.xcd_chunk ? (int)xcd_map(blockIdx.x, (uint32_t)A.xcd_chunk) : (int)blockIdx.x;
   if (w >= A.nwork) return;
   const int gpos = A.work ? (int)A.work[w] : w;
   const uint32_t status = cell_body(A, gpos, S, lane);
@@ -1113,11 +1123,7 @@ __device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<
 template <int VPL, int PPL, int FPL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells_items(BatchArgs B, uint32_t n) {
   __shared__ Lds<VPL, PPL, FPL> S;
-  uint32_t i = blockIdx.x;
-  if (B.xcd_remap) {
-    const uint32_t per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
-    i = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
+  const uint32_t i = B.xcd_chunk ? xcd_map(blockIdx.x, (uint32_t)B.xcd_chunk) : blockIdx.x;
   if (i >= n) return;
   batch_item(B, B.items[B.order ? B.order[i] : i], S, threadIdx.x);
 }
@@ -1166,7 +1172,7 @@ namespace tess {
 
 hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  const uint32_t nb = b.xcd_remap ? (n + 7) & ~7u : n;
+  const uint32_t nb = b.xcd_chunk ? xcd_grid(n, (uint32_t)b.xcd_chunk) : n;
   switch (vpl) {
     case 1:
       hipLaunchKernelGGL((k_cells_items<1, 2, 1>), dim3(nb), dim3(64), 0, s, b, n);
@@ -1198,7 +1204,7 @@ int cells_queue_blocks_per_cu() {
 hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
   if (a.nwork <= 0) return hipSuccess;
   int nb = a.nwork;
-  if (a.xcd_remap) nb = (nb + 7) & ~7;
+  if (a.xcd_chunk) nb = (int)xcd_grid((uint32_t)nb, (uint32_t)a.xcd_chunk);
   switch (vpl) {
     case 1:
       hipLaunchKernelGGL((k_cells<1, 2, 1>), dim3(nb), dim3(64), 0, s, a);

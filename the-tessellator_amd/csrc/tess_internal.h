@@ -68,7 +68,7 @@ struct CellArgs {
   double lo[3], hi[3];     // container box
   const uint32_t* work;    // grid positions to compute (nullptr: 0..nwork-1)
   int nwork;
-  int xcd_remap;           // 1: give each XCD a contiguous range of cells
+  int xcd_chunk;           // > 0: XCDs take runs of this many consecutive cells
   const int64_t* groups;   // nullable
   int64_t target_group;    // < 0: none
   double radius_limit;     // NaN: none (expand_all_in_radius cut-off otherwise)
@@ -98,7 +98,7 @@ struct BatchArgs {
   int chunk;              // items taken per atomic
   uint64_t* redo;         // capacity overflows (nullable)
   uint32_t* nredo;
-  int xcd_remap;          // 1: give each XCD a contiguous range of the list
+  int xcd_chunk;          // > 0: XCDs take runs of this many consecutive items
 };
 
 }  // namespace tess

@@ -50,7 +50,7 @@ class TessError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("cell_density", C.c_double), ("mode", C.c_int32), ("vertex_capacity", C.c_int32),
-                ("collect_stats", C.c_int32), ("reserved", C.c_int32)]
+                ("collect_stats", C.c_int32), ("xcd_chunk", C.c_int32)]
 
 
 class CellStats(C.Structure):
@@ -71,7 +71,7 @@ class Timings(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("first_capacity", C.c_int32), ("xcd_contiguous", C.c_int32), ("redo_chunk", C.c_int32),
+    _fields_ = [("first_capacity", C.c_int32), ("xcd_chunk", C.c_int32), ("redo_chunk", C.c_int32),
                 ("learned", C.c_int32)]
 
 
@@ -263,7 +263,7 @@ class Diagram:
     """Points + container box (tess_diagram, interface.rs:24-84)."""
 
     def __init__(self, ctx: Context, xyz, lo, hi, groups=None, density=1.25, mode=MODE_PLAIN,
-                 vertex_capacity=0, collect_stats=False):
+                 vertex_capacity=0, collect_stats=False, xcd_chunk=0):
         self.ctx, self.lib = ctx, ctx.lib
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
         self.n = xyz.shape[0]
@@ -273,6 +273,7 @@ class Diagram:
         o = Opts()
         self.lib.tess_opts_default(C.byref(o))
         o.cell_density, o.mode, o.vertex_capacity, o.collect_stats = density, mode, vertex_capacity, int(collect_stats)
+        o.xcd_chunk = xcd_chunk
         h = C.c_void_p()
         ctx.check(self.lib.tess_diagram_create(ctx.h, _ptr(xyz), self.n, _ptr(lo), _ptr(hi), _ptr(g),
                                                C.byref(o), C.byref(h)))
