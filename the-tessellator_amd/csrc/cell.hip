@@ -292,12 +292,13 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
   const int ncross = before;
   if (ncross < 3) return CUT_GENERAL;
   if (ncross > 64) return CUT_CAPACITY;
-  // 2. the first ncross free vertex slots (slots holding no vertex now, so
-  //    they can be written before the clip commits) and free pair slots
-  //    (pairs that die are free)
+  // 2. the first ncross free vertex slots (slots holding no vertex, or an
+  //    Outside vertex, which the cut removes: new vertices are written only
+  //    after every lane has read the old ones) and free pair slots (pairs
+  //    that die are free)
   uint64_t vfree[VPL], pfree[PPL];
 #pragma unroll
-  for (int r = 0; r < VPL; ++r) vfree[r] = ~C.VA[r];
+  for (int r = 0; r < VPL; ++r) vfree[r] = ~C.VA[r] | OM[r];
 #pragma unroll
   for (int r = 0; r < PPL; ++r) pfree[r] = ~(C.PA[r] & ~PD[r]);
   if (ncross > popc_multi(vfree) || ncross > popc_multi(pfree)) return CUT_CAPACITY;
@@ -327,6 +328,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
   //    inside) into new vertex slot k, the face maps
   const bool mine = lane < ncross;
   int fin = 0, fout = 0, hin = 0, sv = 0, sp = 0;
+  double nx = 0.0, ny = 0.0, nz = 0.0;
   if (mine) {
     hin = S.cs.f.cpair[lane];  // the half-edge that enters the Outside end
     sv = S.cs.f.vslot[lane];
@@ -337,9 +339,9 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     const double dO = S.vdot[o], dI = S.vdot[i];
     const double t = (off - dO) / (dI - dO);
     const double ox = S.vx[o], oy = S.vy[o], oz = S.vz[o];
-    S.vx[sv] = ox + (S.vx[i] - ox) * t;
-    S.vy[sv] = oy + (S.vy[i] - oy) * t;
-    S.vz[sv] = oz + (S.vz[i] - oz) * t;
+    nx = ox + (S.vx[i] - ox) * t;
+    ny = oy + (S.vy[i] - oy) * t;
+    nz = oz + (S.vz[i] - oz) * t;
     S.fentry[fin] = (uint16_t)hin;
     S.fexit[fout] = (uint16_t)(hin ^ 1);
   }
@@ -356,8 +358,14 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     clash = e_in != hin || x_out != (hin ^ 1);
   }
   if (ballot(clash)) return CUT_GENERAL;
-  // 5. commit: bridge k on the entry face, its flip on the new face
+  // 5. commit: bridge k on the entry face, its flip on the new face.  The
+  //    new vertex may take the slot of an Outside one, so it is written only
+  //    now: every read of the old vertices is done and the general clip can
+  //    no longer be asked to start from the uncut cell.
   if (mine) {
+    S.vx[sv] = nx;
+    S.vy[sv] = ny;
+    S.vz[sv] = nz;
     const int Xe = S.cs.f.vslot[S.cs.f.rank[b >> 1]];
     const int nxt = 2 * S.cs.f.pslot[S.cs.f.rank[a2 >> 1]] + 1;
     const int br = 2 * sp;
