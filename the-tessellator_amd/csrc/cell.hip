@@ -157,6 +157,17 @@ __host__ __device__ __forceinline__ uint32_t xcd_grid(uint32_t n, uint32_t K) {
   return (n + round - 1) / round * round;
 }
 
+// Distance from a site coordinate to grid cell b's slab along one axis, a
+// lower bound for every point binned there: the slab [vmin + b*size,
+// vmin + (b+1)*size] is widened by a millionth of its width, far more than
+// the rounding of the binning and of the distance itself.
+__device__ __forceinline__ double box_gap(int b, double vmin, double size, double site) {
+  const double lo = (vmin + (double)b * size) - site, hi = lo + size, m = size * 1e-6;
+  if (lo - m > 0.0) return lo - m;
+  if (hi + m < 0.0) return -(hi + m);
+  return 0.0;
+}
+
 __device__ __forceinline__ double stop_thr(double r2) { return (4.0 * r2) * (1.0 + 1e-9); }
 
 // A chunk with at least this many staged candidates is pre-tested lane-parallel.
@@ -718,6 +729,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   uint64_t flops = 24 + 40;
   int maxv = 8;
   int e_lastcut = -1;
+  const bool skip_cells = A.target_group < 0;
   int e_stop_final = 0;
   bool done = false;
   const double rlim = A.radius_limit;
@@ -734,6 +746,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     double key = 0.0;
     uint32_t bst = 0;
     int cnt = 0;
+    float ebd2 = 0.0f;  // lower bound of |p - site|^2 over the grid cell's points
     if (have) {
       key = A.tkey[e];
       int ox, oy, oz;
@@ -743,6 +756,9 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         const int bin = (bx * cpd + by) * cpd + bz;
         bst = A.delim[bin];
         cnt = (int)(A.delim[bin + 1] - bst);
+        const double gx = box_gap(bx, G.xmin, G.sx, sx), gy = box_gap(by, G.ymin, G.sy, sy),
+                     gz = box_gap(bz, G.zmin, G.sz, sz);
+        ebd2 = __double2float_rd((gx * gx + gy * gy) + gz * gz);
       }
     }
     const uint64_t sm = ballot(have && (key > stop_thr(r2) || key > rlim));
@@ -752,15 +768,29 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     const int lim = nE < nEnd ? nE : nEnd;
     int e_stop = ebase + lim;  // entries [ebase, e_stop) are reached
     bool stop_here = (sm != 0) && nE <= nEnd;
-    const int c = lane < lim ? cnt : 0;
-    const int incl = wave_incl_scan(c, lane);
-    const int total = rl_i(incl, 63);
-    S.bpref[lane] = (uint32_t)(incl - c);
-    S.bstart[lane] = bst;
-    wave_sync();
-
-    for (int base = 0; base < total && !done; base += 64) {
-      const int k = base + lane;
+    // The chunks walk the candidates of the reached entries in stream order
+    // from a cursor (entry ce, offset co).  Before each chunk, grid cells
+    // that the security ball no longer reaches (lower bound above the stop
+    // threshold) are left out: every candidate in them would be skipped at
+    // its turn, so the result and the counters are unchanged.  With a group
+    // filter the streamed count needs each candidate's group, so nothing is
+    // left out.
+    int ce = 0, co = 0;
+    for (;;) {
+      if (done) break;
+      const double thr_c = stop_thr(r2);
+      int c = 0;
+      if (lane >= ce && ebase + lane < e_stop) {
+        c = cnt - (lane == ce ? co : 0);
+        if (skip_cells && (double)ebd2 > thr_c) c = 0;
+      }
+      const int incl = wave_incl_scan(c, lane);
+      const int total = rl_i(incl, 63);
+      if (total == 0) break;
+      S.bpref[lane] = (uint32_t)(incl - c);
+      S.bstart[lane] = bst + (uint32_t)(lane == ce ? co : 0);
+      wave_sync();
+      const int k = lane;
       bool valid = false;
       int ek = 0;
       uint32_t jidx = 0;
@@ -908,20 +938,30 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         // the smaller cell reaches fewer of the pending candidates
         pv &= ballot(lane < nkeep && S.u.cand.ent[lane] < e_stop && !(S.u.cand.d2[lane] > thr));
       }
-      streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
-      if (stop_here && base + 64 < total) {
-        // candidates of the next chunk start at an entry >= e_stop?
-        const int kn = base + 64;
+      if (!skip_cells) streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
+      // advance the cursor past this chunk's 64 candidates
+      if (total <= 64) {
+        ce = 64;
+        co = 0;
+      } else {
         int lo = 0, hi = lim - 1;
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
-          if ((int)S.bpref[mid] <= kn)
+          if ((int)S.bpref[mid] <= 64)
             lo = mid;
           else
             hi = mid - 1;
         }
-        if (ebase + lo >= e_stop) break;
+        co = 64 - (int)S.bpref[lo] + (lo == ce ? co : 0);
+        ce = lo;
       }
+      wave_sync();
+    }
+    if (skip_cells) {
+      // streamed: every candidate of the reached entries but the site itself
+      const int rc = (ebase + lane < e_stop && lane < lim) ? cnt : 0;
+      const int sc = rl_i(wave_incl_scan(rc, lane), 63);
+      streamed += (uint32_t)(sc - (ebase == 0 ? 1 : 0));
     }
     wave_sync();
     if (stop_here || done) {
