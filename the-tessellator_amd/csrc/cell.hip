@@ -318,19 +318,25 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
   for (int r = FPL - 1; r >= 0; --r)
     if (~C.FA[r]) nf = 64 * r + ctz64(~C.FA[r]);
   if (nf < 0) return CUT_CAPACITY;
+  // the slots taken (the lowest ncross free ones) as masks, from the same ranks
+  uint64_t vnew[VPL], pnew[PPL];
   {
     int bv = 0;
 #pragma unroll
     for (int r = 0; r < VPL; ++r) {
       const int k = bv + mbcnt(vfree[r], lane);
-      if (((vfree[r] >> lane) & 1ull) && k < ncross) S.cs.f.vslot[k] = (uint8_t)(lane + 64 * r);
+      const bool take = ((vfree[r] >> lane) & 1ull) && k < ncross;
+      if (take) S.cs.f.vslot[k] = (uint8_t)(lane + 64 * r);
+      vnew[r] = ballot(take);
       bv += popc(vfree[r]);
     }
     int bp = 0;
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
       const int k = bp + mbcnt(pfree[r], lane);
-      if (((pfree[r] >> lane) & 1ull) && k < ncross) S.cs.f.pslot[k] = (uint8_t)(lane + 64 * r);
+      const bool take = ((pfree[r] >> lane) & 1ull) && k < ncross;
+      if (take) S.cs.f.pslot[k] = (uint8_t)(lane + 64 * r);
+      pnew[r] = ballot(take);
       bp += popc(pfree[r]);
     }
   }
@@ -390,9 +396,6 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     S.he_next[br + 1] = (uint16_t)nxt;
     S.f_start[fin] = (uint16_t)br;
   }
-  uint64_t vnew[VPL], pnew[PPL];
-  lowest_bits(vfree, ncross, vnew);
-  lowest_bits(pfree, ncross, pnew);
 #pragma unroll
   for (int r = 0; r < VPL; ++r) C.VA[r] = (C.VA[r] & ~OM[r]) | vnew[r];
 #pragma unroll
@@ -663,6 +666,27 @@ __device__ double cell_r2(const CellState<VPL, PPL, FPL>& C, int lane) {
   return uni_d(wave_max(m));  // every lane holds the maximum; keep it scalar
 }
 
+// cell_r2 and a vertex slot that attains it.
+template <int VPL, int PPL, int FPL>
+__device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int* slot) {
+  double d[VPL];
+  double m = 0.0;
+#pragma unroll
+  for (int r = 0; r < VPL; ++r) {
+    d[r] = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
+    if ((C.VA[r] >> lane) & 1ull) m = d[r] > m ? d[r] : m;
+  }
+  m = uni_d(wave_max(m));
+  int s = -1;
+#pragma unroll
+  for (int r = VPL - 1; r >= 0; --r) {
+    const uint64_t b = ballot(((C.VA[r] >> lane) & 1ull) && d[r] == m);
+    if (b) s = 64 * r + ctz64(b);
+  }
+  *slot = s;
+  return m;
+}
+
 }  // namespace
 
 // One cell: the site at grid position gpos of the set A.  Returns the
@@ -723,7 +747,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     C.seq = 6;
   }
   wave_sync();
-  double r2 = cell_r2(C, lane);
+  int r2_slot = 0;
+  double r2 = cell_r2_slot(C, lane, &r2_slot);
 
   uint32_t streamed = 0, classified = 0, ncuts = 0, status = 0;
   uint64_t flops = 24 + 40;
@@ -916,6 +941,13 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         int newv = 0;
         const int32_t ctag = uni_i(S.u.cand.tag[q]);
         const int ec = uni_i(S.u.cand.ent[q]);
+        uint64_t kept[VPL];  // vertices the cut keeps
+        bool max_cut = false;
+#pragma unroll
+        for (int r = 0; r < VPL; ++r) {
+          kept[r] = C.VA[r] & ~OM[r];
+          if ((r2_slot >> 6) == r) max_cut = (OM[r] >> (r2_slot & 63)) & 1ull;
+        }
         int rc = clip_fast(S, C, OM, coff, ctag, lane, &newv);
         if (rc == CUT_GENERAL) rc = clip(S, C, OM, coff, ctag, lane, &newv);
         if (rc != CUT_OK) {
@@ -927,7 +959,21 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         nv = popc_multi(C.VA);
         if (nv > maxv) maxv = nv;
         flops += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
-        r2 = cell_r2(C, lane);
+        // the largest |v|^2 changes only if its vertex was cut off or a new
+        // vertex exceeds it (new vertices lie on old edges, but rounding
+        // may put one an ulp beyond)
+        bool r2_same = !max_cut;
+        if (r2_same) {
+          uint64_t big = 0;
+#pragma unroll
+          for (int r = 0; r < VPL; ++r) {
+            const bool isnew = ((C.VA[r] & ~kept[r]) >> lane) & 1ull;
+            const double d = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
+            big |= ballot(isnew && d > r2);
+          }
+          r2_same = big == 0;
+        }
+        if (!r2_same) r2 = cell_r2_slot(C, lane, &r2_slot);
         e_lastcut = ec;
         const double thr = stop_thr(r2);
         const uint64_t sm2 = ballot(lane < lim && ebase + lane > e_lastcut && (key > thr || key > rlim));
