@@ -973,16 +973,21 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           }
           r2_same = big == 0;
         }
-        if (!r2_same) r2 = cell_r2_slot(C, lane, &r2_slot);
         e_lastcut = ec;
-        const double thr = stop_thr(r2);
-        const uint64_t sm2 = ballot(lane < lim && ebase + lane > e_lastcut && (key > thr || key > rlim));
-        if (sm2) {
-          e_stop = ebase + ctz64(sm2);
-          stop_here = true;
+        // With an unchanged radius the stop entry and the pending set stay as
+        // they are: the entries between the last cut and the stop entry all
+        // have keys within the unchanged threshold.
+        if (!r2_same) {
+          r2 = cell_r2_slot(C, lane, &r2_slot);
+          const double thr = stop_thr(r2);
+          const uint64_t sm2 = ballot(lane < lim && ebase + lane > e_lastcut && (key > thr || key > rlim));
+          if (sm2) {
+            e_stop = ebase + ctz64(sm2);
+            stop_here = true;
+          }
+          // the smaller cell reaches fewer of the pending candidates
+          pv &= ballot(lane < nkeep && S.u.cand.ent[lane] < e_stop && !(S.u.cand.d2[lane] > thr));
         }
-        // the smaller cell reaches fewer of the pending candidates
-        pv &= ballot(lane < nkeep && S.u.cand.ent[lane] < e_stop && !(S.u.cand.d2[lane] > thr));
       }
       if (!skip_cells) streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
       // advance the cursor past this chunk's 64 candidates
