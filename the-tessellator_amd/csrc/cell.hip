@@ -278,6 +278,11 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
   int before = 0;
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
+    if (r > 0 && C.PA[r] == 0) {  // pair slots fill from the bottom
+      CM[r] = 0;
+      PD[r] = 0;
+      continue;
+    }
     const int p = lane + 64 * r;
     const bool al = (C.PA[r] >> lane) & 1ull;
     uint32_t c0 = C_IN, c1 = C_IN;
@@ -416,6 +421,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     wave_sync();
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
+      if (r > 0 && C.PA[r] == 0) continue;
       const int q = lane + 64 * r;
       if ((C.PA[r] >> lane) & 1ull) {
         S.fmark[S.he_face[2 * q]] = 1;
