@@ -719,6 +719,9 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 #pragma unroll
     for (int r = 0; r < VPL; ++r) {
       const int v = lane + 64 * r;
+      C.rx[r] = 0.0;  // slots without a vertex hold zeros (tests on them are masked)
+      C.ry[r] = 0.0;
+      C.rz[r] = 0.0;
       if (v < 8) {
         const double vx = (v == 1 || v == 2 || v == 5 || v == 6) ? x1 : x0;
         const double vy = (v >= 4) ? y1 : y0;
@@ -911,36 +914,28 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         const double cnz = uni_d(S.u.cand.n[q][2]), coff = uni_d(S.u.cand.n[q][3]);
         classified++;
         flops += 13 + 6ull * (uint64_t)nv;
+        // Outside test only; the three-way classification is needed only
+        // when the candidate cuts (same arithmetic both times)
         uint64_t OM[VPL];
-        uint8_t cv[VPL];
         double dv[VPL];
         bool any = false;
 #pragma unroll
         for (int r = 0; r < VPL; ++r) {
           if (r > 0 && C.VA[r] == 0) {  // vertex slots fill from the bottom
-            cv[r] = C_IN;
             dv[r] = 0.0;
             OM[r] = 0;
             continue;
           }
-          const bool al = (C.VA[r] >> lane) & 1ull;
-          uint8_t cc = C_IN;
-          double d = 0.0;
-          if (al) {
-            d = (cnx * C.rx[r] + cny * C.ry[r]) + cnz * C.rz[r];
-            const double sd = d - coff;
-            cc = sd > kTol ? C_OUT : (sd < -kTol ? C_IN : C_INC);
-          }
-          cv[r] = cc;
-          dv[r] = d;
-          OM[r] = ballot(al && cc == C_OUT);
+          dv[r] = (cnx * C.rx[r] + cny * C.ry[r]) + cnz * C.rz[r];
+          OM[r] = ballot(dv[r] - coff > kTol) & C.VA[r];
           any = any || OM[r] != 0;
         }
         if (!any) continue;
 #pragma unroll
         for (int r = 0; r < VPL; ++r)
           if ((C.VA[r] >> lane) & 1ull) {
-            S.vcls[lane + 64 * r] = cv[r];
+            const double sd = dv[r] - coff;
+            S.vcls[lane + 64 * r] = sd > kTol ? C_OUT : (sd < -kTol ? C_IN : C_INC);
             S.vdot[lane + 64 * r] = dv[r];
           }
         wave_sync();
