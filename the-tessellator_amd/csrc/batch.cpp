@@ -599,9 +599,9 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
   }
   std::vector<tess_batch_config> cand;
   for (int cap : {64, 128})
-    for (int xcd : {0, 512, 4096}) cand.push_back({cap, xcd, 4, 0});
+    for (int xcd : {0, 256, 4096}) cand.push_back({cap, xcd, 4, 0});
   if (can_learn)
-    for (int xcd : {0, 512}) cand.push_back({64, xcd, 4, 1});
+    for (int xcd : {0, 256}) cand.push_back({64, xcd, 4, 1});
   double best_ms = 1e300;
   tess_batch_config bk = b->cfg;
   for (const tess_batch_config& k : cand) {
