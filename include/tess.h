@@ -178,6 +178,50 @@ size_t tess_cell_face_vertices(const tess_cell* c, size_t face, double* xyz, siz
 /* Cell::compute_vertices (interface.rs:368-370), site-relative. */
 size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
 
+/* ---- radius queries (celery.rs:700-855, interface.rs:348-365) ---------
+ * Many queries of one kind over a diagram's grid (the grid is built first
+ * if no compute has built it), with results as CSR: the ids of query i are
+ * ids[offsets[i] .. offsets[i + 1]), in the reference's order.
+ *  TESS_QUERY_CELLS        Celery::find_cells_in_radius (celery.rs:753-797):
+ *                          grid cell ids (i cpd + j) cpd + k of the cells in
+ *                          the clamped index box whose gap distance from the
+ *                          query's cell is <= radius, in i, j, k loop order.
+ *  TESS_QUERY_CELL_RADIUS  find_neighbors_in_cell_radius (:802-819): the
+ *                          points of those cells, cell by cell, each cell's
+ *                          points in grid (sorted-index) order.
+ *  TESS_QUERY_REAL_RADIUS  find_neighbors_in_real_radius (:825-855): the same
+ *                          walk, keeping points with squared distance
+ *                          <= radius^2.
+ *  TESS_QUERY_CLOUD        Cell::compute_neighbor_cloud (interface.rs:348-365):
+ *                          ExpandingSearch::expand_all_in_radius from the
+ *                          query position (celery.rs:1023-1075: bins in
+ *                          search order while the squared gap distance is
+ *                          <= radius, compared unsquared as there), then the
+ *                          target-group filter when target_group >= 0.
+ * Point ids are original indices.  xyz: nq host AoS positions (any position:
+ * its grid cell is clamped as in celery.rs:269-282).  radius: nradius = 1
+ * (one radius for all) or nq values.  target_group >= 0 needs
+ * TESS_QUERY_CLOUD and a diagram with groups.  A NaN radius lists nothing
+ * for the box kinds and every point for the cloud, as in the reference. */
+enum {
+  TESS_QUERY_CELLS = 0,
+  TESS_QUERY_CELL_RADIUS = 1,
+  TESS_QUERY_REAL_RADIUS = 2,
+  TESS_QUERY_CLOUD = 3
+};
+typedef struct tess_query tess_query;
+tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, size_t nq, const double* radius,
+                               size_t nradius, int64_t target_group, tess_query** out);
+size_t tess_query_count(const tess_query* q);   /* number of queries */
+uint64_t tess_query_total(const tess_query* q); /* ids over all queries */
+/* offsets: nq + 1 entries; ids: tess_query_total entries (either may be NULL) */
+tess_status tess_query_copy(const tess_query* q, uint64_t* offsets, uint32_t* ids);
+/* Device pointers owned by q (valid until tess_query_destroy). */
+tess_status tess_query_device(const tess_query* q, const uint64_t** offsets, const uint32_t** ids);
+/* Device time of the query passes (count, scan, fill), ms. */
+double tess_query_ms(const tess_query* q);
+void tess_query_destroy(tess_query* q);
+
 /* ---- learned search-radius model (no counterpart in the reference) -----
  * A 27 -> 64 -> 64 -> 2 ReLU network.  Input: log(1 + count) of the 3x3x3
  * grid cells around a point's grid cell (bf16, padded to TESS_MLP_IN).

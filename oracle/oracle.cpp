@@ -722,6 +722,64 @@ void orc_grid_build(const double* x, const double* y, const double* z, size_t n,
   for (size_t i = 0; i < n; ++i) sorted[cnt[bins[i]]++] = (uint32_t)i;
 }
 
+// celery.rs:708-743.  sq(i sx) + sq(j sy) + sq(k sz) of the per-axis cell
+// gaps max(0, |a - b| - 1), against radius * radius.
+int orc_check_cell_in_range(const orc_grid* g, double x, double y, double z, double radius, int64_t i, int64_t j,
+                            int64_t k) {
+  int64_t xi, yi, zi;
+  bin_of(g, x, y, z, &xi, &yi, &zi);
+  auto gap = [](int64_t a, int64_t b) {
+    const int64_t d = (a > b ? a - b : b - a) - 1;
+    return d > 0 ? d : 0;
+  };
+  auto sq = [](double t) { return t * t; };
+  const double ds = sq((double)gap(xi, i) * g->sx) + sq((double)gap(yi, j) * g->sy) + sq((double)gap(zi, k) * g->sz);
+  return ds <= radius * radius;
+}
+
+// celery.rs:682-697: the first argument only when strictly smaller / larger.
+static inline double min_float(double a, double b) { return a < b ? a : b; }
+static inline double max_float(double a, double b) { return a > b ? a : b; }
+
+// celery.rs:753-797.
+size_t orc_find_cells_in_radius(const orc_grid* g, double x, double y, double z, double radius, uint32_t* out) {
+  int64_t i0, j0, k0, i1, j1, k1;
+  bin_of(g, max_float(x - radius, g->xmin), max_float(y - radius, g->ymin), max_float(z - radius, g->zmin), &i0, &j0,
+         &k0);
+  bin_of(g, min_float(x + radius, g->xmax), min_float(y + radius, g->ymax), min_float(z + radius, g->zmax), &i1, &j1,
+         &k1);
+  size_t c = 0;
+  for (int64_t i = i0; i <= i1; ++i)
+    for (int64_t j = j0; j <= j1; ++j)
+      for (int64_t k = k0; k <= k1; ++k)
+        if (orc_check_cell_in_range(g, x, y, z, radius, i, j, k)) {
+          if (out) out[c] = (uint32_t)((i * g->cpd + j) * g->cpd + k);  // celery.rs:317-325
+          ++c;
+        }
+  return c;
+}
+
+// celery.rs:802-819 (real == 0) and :825-855 (real != 0).
+size_t orc_find_neighbors(const double* px, const double* py, const double* pz, const orc_grid* g,
+                          const uint32_t* sorted, const uint64_t* delim, double x, double y, double z, double radius,
+                          int real, uint32_t* out) {
+  std::vector<uint32_t> cells(orc_find_cells_in_radius(g, x, y, z, radius, nullptr));
+  orc_find_cells_in_radius(g, x, y, z, radius, cells.data());
+  size_t c = 0;
+  const double r2 = radius * radius;
+  for (uint32_t b : cells)
+    for (uint64_t s = delim[b]; s < delim[b + 1]; ++s) {
+      const uint32_t p = sorted[s];
+      if (real) {
+        const double dx = x - px[p], dy = y - py[p], dz = z - pz[p];  // celery.rs:700-702
+        if (!((dx * dx + dy * dy) + dz * dz <= r2)) continue;
+      }
+      if (out) out[c] = p;
+      ++c;
+    }
+  return c;
+}
+
 // Position of an offset in the generation loops of celery.rs:437-673.
 uint64_t orc_generation_index(int64_t cpd, int32_t ox, int32_t oy, int32_t oz) {
   const uint64_t M = (uint64_t)(cpd - 1);
@@ -1030,6 +1088,48 @@ size_t orc_expand_all(const double* x, const double* y, const double* z, size_t 
     for (uint64_t k = G.delim[b]; k < G.delim[b + 1]; ++k) out[c++] = G.sorted[k];
   }
   return c;
+}
+
+void orc_queries(int kind, const double* x, const double* y, const double* z, size_t n, double density,
+                 const int64_t* groups, int64_t target_group, const double* q, size_t nq, const double* radius,
+                 uint64_t* offsets, uint32_t* out) {
+  Grid G;
+  grid_setup(G, x, y, z, n, density, kind == 3);
+  const int64_t cpd = G.g.cpd;
+  uint64_t c = 0;
+  std::vector<uint32_t> tmp;
+  for (size_t a = 0; a < nq; ++a) {
+    offsets[a] = c;
+    const double qx = q[3 * a], qy = q[3 * a + 1], qz = q[3 * a + 2], r = radius[a];
+    if (kind == 0) {
+      const size_t m = orc_find_cells_in_radius(&G.g, qx, qy, qz, r, nullptr);
+      if (out) orc_find_cells_in_radius(&G.g, qx, qy, qz, r, out + c);
+      c += m;
+    } else if (kind == 1 || kind == 2) {
+      const size_t m = orc_find_neighbors(x, y, z, &G.g, G.sorted.data(), G.delim.data(), qx, qy, qz, r, kind == 2,
+                                          nullptr);
+      if (out)
+        orc_find_neighbors(x, y, z, &G.g, G.sorted.data(), G.delim.data(), qx, qy, qz, r, kind == 2, out + c);
+      c += m;
+    } else {
+      // celery.rs:1023-1075 from the bin of the query position.
+      int64_t ix, iy, iz;
+      bin_of(&G.g, qx, qy, qz, &ix, &iy, &iz);
+      for (size_t t = 0; t < G.key.size(); ++t) {
+        if (G.key[t] > r) break;
+        const int64_t bx = ix + G.off[3 * t], by = iy + G.off[3 * t + 1], bz = iz + G.off[3 * t + 2];
+        if (bx < 0 || bx >= cpd || by < 0 || by >= cpd || bz < 0 || bz >= cpd) continue;
+        const uint64_t b = (uint64_t)((bx * cpd + by) * cpd + bz);
+        for (uint64_t k = G.delim[b]; k < G.delim[b + 1]; ++k) {
+          const uint32_t p = G.sorted[k];
+          if (target_group >= 0 && groups[p] != target_group) continue;  // interface.rs:358-361
+          if (out) out[c] = p;
+          ++c;
+        }
+      }
+    }
+  }
+  offsets[nq] = c;
 }
 
 orc_poly* orc_poly_new(double xmin, double ymin, double zmin, double xmax, double ymax, double zmax) {

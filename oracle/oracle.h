@@ -67,6 +67,29 @@ uint64_t orc_generation_index(int64_t cpd, int32_t ox, int32_t oy, int32_t oz);
 
 /* Candidate stream of a site: expand_all_no_radius (max_radius NaN, all n
  * points) or expand_all_in_radius(max_radius). Returns the count. */
+/* Radius queries (celery.rs:700-855) on a built grid.  check_cell_in_range
+ * (celery.rs:708-743): is grid cell (i, j, k) within radius of the cell of
+ * (x, y, z), counting touching cells as distance 0.  find_cells_in_radius
+ * (celery.rs:753-797): cell ids in i, j, k loop order over the clamped box.
+ * find_neighbors (celery.rs:802-855): the points of those cells in cell
+ * order then sorted order; real != 0 keeps only points with squared
+ * distance <= radius^2.  out may be NULL (count only). */
+int orc_check_cell_in_range(const orc_grid* g, double x, double y, double z, double radius, int64_t i, int64_t j,
+                            int64_t k);
+size_t orc_find_cells_in_radius(const orc_grid* g, double x, double y, double z, double radius, uint32_t* out);
+size_t orc_find_neighbors(const double* px, const double* py, const double* pz, const orc_grid* g,
+                          const uint32_t* sorted, const uint64_t* delim, double x, double y, double z, double radius,
+                          int real, uint32_t* out);
+/* Many radius queries over one grid built from the points (celery.rs:231-250):
+ * kind 0 find_cells_in_radius, 1 find_neighbors_in_cell_radius, 2
+ * find_neighbors_in_real_radius, 3 Cell::compute_neighbor_cloud
+ * (interface.rs:348-365: expand_all_in_radius from the query position, then
+ * the target-group filter when target_group >= 0; groups may be NULL then).
+ * q: nq AoS positions; radius: nq radii.  offsets (nq + 1) are always
+ * written; out (offsets[nq] entries) only when not NULL. */
+void orc_queries(int kind, const double* x, const double* y, const double* z, size_t n, double density,
+                 const int64_t* groups, int64_t target_group, const double* q, size_t nq, const double* radius,
+                 uint64_t* offsets, uint32_t* out);
 size_t orc_expand_all(const double* x, const double* y, const double* z, size_t n, double density,
                       double sxp, double syp, double szp, double max_radius, uint32_t* out);
 

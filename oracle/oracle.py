@@ -66,6 +66,10 @@ def lib():
             "orc_generation_index": (C.c_uint64, [C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
             "orc_expand_all": (sz, [dp, dp, dp, sz, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     dp]),
+            "orc_check_cell_in_range": (C.c_int, [C.POINTER(Grid)] + [C.c_double] * 4 + [C.c_int64] * 3),
+            "orc_find_cells_in_radius": (sz, [C.POINTER(Grid)] + [C.c_double] * 4 + [dp]),
+            "orc_find_neighbors": (sz, [dp, dp, dp, C.POINTER(Grid), dp, dp] + [C.c_double] * 4 + [C.c_int, dp]),
+            "orc_queries": (None, [C.c_int, dp, dp, dp, sz, C.c_double, dp, C.c_int64, dp, sz, dp, dp, dp]),
             "orc_poly_new": (vp, [C.c_double] * 6),
             "orc_poly_new_faces": (vp, [C.c_double] * 6),
             "orc_poly_free": (None, [vp]),
@@ -187,6 +191,48 @@ def expand_all(xyz, site, density=1.25, max_radius=None):
     m = lib().orc_expand_all(_p(x), _p(y), _p(z), len(x), density, float(site[0]), float(site[1]),
                              float(site[2]), r, _p(out))
     return out[:m]
+
+
+# ---- radius queries (celery.rs:700-855, interface.rs:348-365) ------------------
+QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
+
+
+def check_cell_in_range(g, x, y, z, radius, i, j, k):
+    return bool(lib().orc_check_cell_in_range(C.byref(g), x, y, z, radius, i, j, k))
+
+
+def find_cells_in_radius(g, x, y, z, radius):
+    m = lib().orc_find_cells_in_radius(C.byref(g), x, y, z, radius, None)
+    out = np.empty(m, np.uint32)
+    lib().orc_find_cells_in_radius(C.byref(g), x, y, z, radius, _p(out))
+    return out
+
+
+def find_neighbors(xyz, g, srt, dl, x, y, z, radius, real=False):
+    """find_neighbors_in_cell_radius (real False) / _in_real_radius (True)."""
+    px, py, pz = soa(xyz)
+    srt = np.ascontiguousarray(srt, np.uint32)
+    dl = np.ascontiguousarray(dl, np.uint64)
+    args = (_p(px), _p(py), _p(pz), C.byref(g), _p(srt), _p(dl), x, y, z, radius, int(real))
+    m = lib().orc_find_neighbors(*args, None)
+    out = np.empty(m, np.uint32)
+    lib().orc_find_neighbors(*args, _p(out))
+    return out
+
+
+def queries(kind, xyz, q, radius, density=1.25, groups=None, target_group=-1):
+    """Many queries of one kind over the grid of xyz: (offsets (nq+1), ids)."""
+    px, py, pz = soa(xyz)
+    q = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1, 3))
+    nq = len(q)
+    r = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, np.float64), (nq,)))
+    g = None if groups is None else np.ascontiguousarray(groups, np.int64)
+    off = np.empty(nq + 1, np.uint64)
+    args = [kind, _p(px), _p(py), _p(pz), len(px), density, _p(g), int(target_group), _p(q), nq, _p(r), _p(off)]
+    lib().orc_queries(*args, None)
+    ids = np.empty(int(off[-1]), np.uint32)
+    lib().orc_queries(*args, _p(ids))
+    return off, ids
 
 
 # ---- polyhedron (polyhedron.rs) ----------------------------------------------

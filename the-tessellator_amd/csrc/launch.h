@@ -67,6 +67,11 @@ hipError_t launch_mark_status(const uint32_t* list, int m, const uint32_t* sorte
                               tess_cell_stats* stats, uint32_t flag, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t m, uint32_t v, hipStream_t s);
 
+// query.hip: radius queries, one wavefront per query (count, then fill at
+// offsets from launch_face_offsets over the counts)
+hipError_t launch_query_count(const QueryArgs& a, hipStream_t s);
+hipError_t launch_query_fill(const QueryArgs& a, hipStream_t s);
+
 // mlp.hip: learned search-radius model (bf16 MFMA, fp32 accumulate)
 hipError_t launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);
 hipError_t launch_features(const uint32_t* bins, const uint32_t* delim, int cpd, int n, uint16_t* feat,

@@ -1,0 +1,165 @@
+// query.cpp -- radius queries over a diagram's grid on the device
+// (Celery::find_cells_in_radius / find_neighbors_in_{cell,real}_radius,
+// celery.rs:700-855; Cell::compute_neighbor_cloud, interface.rs:348-365).
+#include <cmath>
+#include <limits>
+#include <memory>
+
+#include "launch.h"
+#include "runtime.h"
+
+struct tess_query {
+  tess_ctx* ctx = nullptr;
+  size_t nq = 0;
+  uint64_t total = 0;
+  double ms = 0;
+  DevBuf<uint64_t> offsets;
+  DevBuf<uint32_t> ids;
+};
+
+namespace {
+
+// Shell-table extent for neighbour clouds: the smallest L = kTableL * 2^k
+// whose cut-off min_a (L s_a)^2 exceeds every finite radius, so that the
+// entries with key <= radius are a prefix of the table; NaN and +inf radii
+// (no stop, celery.rs:1036) and large radii take the full order (L = cpd - 1).
+int cloud_table_extent(int64_t cpd, const GridDev& G, const double* r, size_t m) {
+  const int full = (int)(cpd - 1);
+  double rmax = -std::numeric_limits<double>::infinity();
+  for (size_t i = 0; i < m; ++i) {
+    if (!(r[i] < std::numeric_limits<double>::infinity())) return full;
+    rmax = std::max(rmax, r[i]);
+  }
+  auto sq = [](double t) { return t * t; };
+  auto kcut = [&](int L) { return std::min(sq(L * G.sx), std::min(sq(L * G.sy), sq(L * G.sz))); };
+  int L = kTableL;
+  while (L < full && !(kcut(L) > rmax)) L *= 2;
+  return L < full ? L : full;
+}
+
+struct Events {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  Events() {
+    for (auto& x : e) (void)hipEventCreate(&x);
+  }
+  ~Events() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, size_t nq, const double* radius,
+                               size_t nradius, int64_t target_group, tess_query** out) {
+  if (!d || !out) return TESS_E_INVALID;
+  *out = nullptr;
+  tess_ctx* c = d->ctx;
+  if (kind < TESS_QUERY_CELLS || kind > TESS_QUERY_CLOUD) return c->fail(TESS_E_INVALID, "unknown query kind");
+  if (nq && (!xyz || !radius)) return c->fail(TESS_E_INVALID, "null query positions or radii");
+  if (nq && nradius != 1 && nradius != nq) return c->fail(TESS_E_INVALID, "nradius must be 1 or the query count");
+  if (nq > (size_t)std::numeric_limits<int>::max() - 4) return c->fail(TESS_E_INVALID, "too many queries");
+  if (target_group >= 0 && kind != TESS_QUERY_CLOUD)
+    return c->fail(TESS_E_INVALID, "target_group applies to TESS_QUERY_CLOUD only");
+  if (target_group >= 0 && !d->have_groups) return c->fail(TESS_E_INVALID, "diagram has no groups");
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  tess_status st;
+  if (!d->binned && (st = bin(d)) != TESS_OK) return st;
+  std::unique_ptr<tess_query> q(new (std::nothrow) tess_query);
+  if (!q) return c->fail(TESS_E_NOMEM, "host allocation failed");
+  q->ctx = c;
+  q->nq = nq;
+  if (q->offsets.alloc(nq + 1) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if (nq == 0) {
+    const uint64_t zero = 0;
+    if ((st = c->hip(hipMemcpy(q->offsets.p, &zero, 8, hipMemcpyHostToDevice), "upload")) != TESS_OK) return st;
+    *out = q.release();
+    return TESS_OK;
+  }
+  const GridDev& G = d->hgrid;
+  const Table* t = nullptr;
+  if (kind == TESS_QUERY_CLOUD) {
+    const int L = cloud_table_extent(d->cpd, G, radius, nradius);
+    if (L > 511) return c->fail(TESS_E_INVALID, "radius spans more than 511 grid cells");
+    if (!(t = c->table(d->cpd, G.sx, G.sy, G.sz, L, &st))) return st;
+  }
+  DevBuf<double> qd, rd;
+  DevBuf<uint32_t> cnt;
+  DevBuf<uint64_t> sums;
+  if (qd.alloc(3 * nq) != hipSuccess || rd.alloc(nradius) != hipSuccess || cnt.alloc(nq) != hipSuccess ||
+      sums.alloc((nq + 1023) / 1024 + 1) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if ((st = c->hip(hipMemcpyAsync(qd.p, xyz, 24 * nq, hipMemcpyHostToDevice, s), "upload")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(rd.p, radius, 8 * nradius, hipMemcpyHostToDevice, s), "upload")) != TESS_OK)
+    return st;
+  QueryArgs a{};
+  a.g = G;
+  a.sorted = d->sorted.p;
+  a.delim = d->delim.p;
+  a.sx = d->sx.p;
+  a.sy = d->sy.p;
+  a.sz = d->sz.p;
+  a.groups = target_group >= 0 ? d->groups.p : nullptr;
+  a.target_group = target_group;
+  a.q = qd.p;
+  a.radius = rd.p;
+  a.rstride = nradius == 1 ? 0 : 1;
+  a.nq = (int)nq;
+  a.kind = kind;
+  a.key = t ? t->key.p : nullptr;
+  a.off = t ? t->off.p : nullptr;
+  a.tlen = t ? t->len : 0;
+  a.count = cnt.p;
+  Events ev;
+  if ((st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
+      (st = c->hip(launch_query_count(a, s), "query count")) != TESS_OK ||
+      (st = c->hip(launch_face_offsets(cnt.p, (int)nq, q->offsets.p, sums.p, s), "query scan")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&q->total, q->offsets.p + nq, 8, hipMemcpyDeviceToHost, s), "download")) !=
+          TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "query count")) != TESS_OK)
+    return st;
+  if (q->ids.alloc(q->total) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+  a.offsets = q->offsets.p;
+  a.ids = q->ids.p;
+  float ms = 0;
+  if ((st = c->hip(launch_query_fill(a, s), "query fill")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "query fill")) != TESS_OK ||
+      (st = c->hip(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "event")) != TESS_OK)
+    return st;
+  q->ms = ms;
+  *out = q.release();
+  return TESS_OK;
+}
+
+size_t tess_query_count(const tess_query* q) { return q ? q->nq : 0; }
+uint64_t tess_query_total(const tess_query* q) { return q ? q->total : 0; }
+double tess_query_ms(const tess_query* q) { return q ? q->ms : 0.0; }
+
+tess_status tess_query_copy(const tess_query* q, uint64_t* offsets, uint32_t* ids) {
+  if (!q) return TESS_E_INVALID;
+  (void)hipSetDevice(q->ctx->device);
+  hipError_t e = hipSuccess;
+  if (offsets) e = hipMemcpy(offsets, q->offsets.p, 8 * (q->nq + 1), hipMemcpyDeviceToHost);
+  if (ids && q->total && e == hipSuccess) e = hipMemcpy(ids, q->ids.p, 4 * q->total, hipMemcpyDeviceToHost);
+  return q->ctx->hip(e, "download");
+}
+
+tess_status tess_query_device(const tess_query* q, const uint64_t** offsets, const uint32_t** ids) {
+  if (!q) return TESS_E_INVALID;
+  if (offsets) *offsets = q->offsets.p;
+  if (ids) *ids = q->total ? q->ids.p : nullptr;
+  return TESS_OK;
+}
+
+void tess_query_destroy(tess_query* q) {
+  if (!q) return;
+  (void)hipSetDevice(q->ctx->device);
+  delete q;
+}
+
+}  // extern "C"

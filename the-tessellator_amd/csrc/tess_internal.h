@@ -101,4 +101,25 @@ struct BatchArgs {
   int xcd_chunk;          // > 0: XCDs take runs of this many consecutive items
 };
 
+// Radius queries over one grid (query.hip).
+struct QueryArgs {
+  GridDev g;
+  const uint32_t* sorted;  // grid-ordered point ids [n]
+  const uint32_t* delim;   // [cpd^3 + 1]
+  const double *sx, *sy, *sz;  // gathered coordinates (grid order)
+  const int64_t* groups;   // by point id; null: no group filter
+  int64_t target_group;
+  const double* q;         // query positions, AoS [3 nq]
+  const double* radius;    // radius of query i: radius[i * rstride]
+  int rstride;
+  int nq;
+  int kind;                // 0 cells, 1 cell radius, 2 real radius, 3 neighbour cloud
+  const double* key;       // shell table (kind 3)
+  const int32_t* off;
+  int tlen;
+  uint32_t* count;         // count pass: results per query
+  const uint64_t* offsets; // fill pass: CSR offsets [nq + 1]
+  uint32_t* ids;
+};
+
 }  // namespace tess

@@ -37,8 +37,10 @@ EXPORTS = [
     "tess_diagram_compute_scheduled", "tess_diagram_features", "tess_mlp_forward", "tess_mlp_gradient",
     "tess_mfma_probe", "tess_batch_create", "tess_batch_destroy", "tess_batch_sets", "tess_batch_set_config",
     "tess_batch_get_config", "tess_batch_compute", "tess_batch_tune", "tess_batch_total_faces",
-    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith",
+    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith", "tess_diagram_query", "tess_query_count",
+    "tess_query_total", "tess_query_copy", "tess_query_device", "tess_query_ms", "tess_query_destroy",
 ]
+QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 MLP_IN, MLP_PARAMS = 32, 7312
 
 
@@ -144,6 +146,13 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_batch_copy_cells": (C.c_int, [vp, sz, dp, dp, dp, dp, dp]),
         "tess_batch_timings": (C.c_int, [vp, C.POINTER(Timings)]),
         "tess_diag_arith": (C.c_int, [vp, dp, dp, sz, dp, dp, dp]),
+        "tess_diagram_query": (C.c_int, [vp, C.c_int, dp, sz, dp, sz, C.c_int64, C.POINTER(vp)]),
+        "tess_query_count": (sz, [vp]),
+        "tess_query_total": (C.c_uint64, [vp]),
+        "tess_query_copy": (C.c_int, [vp, dp, dp]),
+        "tess_query_device": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
+        "tess_query_ms": (C.c_double, [vp]),
+        "tess_query_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -267,6 +276,7 @@ class Diagram:
         self.ctx, self.lib = ctx, ctx.lib
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
         self.n = xyz.shape[0]
+        self.xyz = xyz
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
         g = None if groups is None else np.ascontiguousarray(groups, np.int64)
@@ -354,6 +364,41 @@ class Diagram:
         out = np.empty((self.n, MLP_IN), np.uint16)
         self.ctx.check(self.lib.tess_diagram_features(self.h, _ptr(out)))
         return out
+
+    def query(self, kind, positions, radius, target_group=None):
+        """Radius queries (tess_diagram_query): CSR offsets [nq + 1] and ids;
+        the ids of query i are ids[offsets[i]:offsets[i + 1]].  radius: one
+        value or one per query.  The device time is kept in last_query_ms."""
+        q = np.ascontiguousarray(np.asarray(positions, np.float64).reshape(-1, 3))
+        r = np.ascontiguousarray(np.atleast_1d(np.asarray(radius, np.float64)))
+        g = -1 if target_group is None else int(target_group)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.tess_diagram_query(self.h, kind, _ptr(q), len(q), _ptr(r), r.size, g, C.byref(h)))
+        try:
+            off = np.empty(len(q) + 1, np.uint64)
+            ids = np.empty(self.lib.tess_query_total(h), np.uint32)
+            self.ctx.check(self.lib.tess_query_copy(h, _ptr(off), _ptr(ids)))
+            self.last_query_ms = self.lib.tess_query_ms(h)
+        finally:
+            self.lib.tess_query_destroy(h)
+        return off, ids
+
+    def find_cells_in_radius(self, x, y, z, radius):
+        """Celery::find_cells_in_radius (celery.rs:753-797): grid cell ids."""
+        return self.query(QUERY_CELLS, [x, y, z], radius)[1]
+
+    def find_neighbors_in_cell_radius(self, x, y, z, radius):
+        """Celery::find_neighbors_in_cell_radius (celery.rs:802-819)."""
+        return self.query(QUERY_CELL_RADIUS, [x, y, z], radius)[1]
+
+    def find_neighbors_in_real_radius(self, x, y, z, radius):
+        """Celery::find_neighbors_in_real_radius (celery.rs:825-855)."""
+        return self.query(QUERY_REAL_RADIUS, [x, y, z], radius)[1]
+
+    def neighbor_cloud(self, index, radius, target_group=None):
+        """Cell::compute_neighbor_cloud (interface.rs:348-365) of the cell of
+        point `index` (its position is the point's)."""
+        return self.query(QUERY_CLOUD, self.xyz[index], radius, target_group)[1]
 
     def get_cell_at_index(self, index, search_radius=None, target_group=None):
         """interface.rs:186-208 + Cell::compute_voronoi_cell."""
