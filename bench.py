@@ -21,6 +21,10 @@ Workloads (--workload):
   batch512     512 independent sets, N log-uniform on [1e3, 2e5], box sides
                uniform on [0.5, 2], uniform points (seeds 3000 + i); batched
                path vs the plain kernel set by set.
+  queries1m    radius queries (find_neighbors_in_real_radius, r = 0.01) at
+               the 1,000,000 points of the clustered set (seed 25, grid of
+               about 5 points per cell), positions and radii resident in
+               HBM, results left in HBM; reports queries/s (its own metric).
 """
 import argparse
 import json
@@ -53,6 +57,9 @@ WORKLOADS = {
                       points=1_000_000, seed=1, density=5.0, kind="uniform"),
     "batch512": dict(desc="batched kernel, 512 uniform sets, N log-uniform [1e3, 2e5], box sides U[0.5, 2]",
                      sets=512, seed=3000, density=5.0, kind="batch"),
+    "queries1m": dict(desc="real-radius neighbour queries (r = 0.01) at the 1,000,000 points of the clustered set "
+                           "(64 Gaussians sigma 0.02 + 20% uniform, seed 25), f64",
+                      points=1_000_000, seed=25, density=5.0, kind="queries", radius=0.01),
 }
 
 
@@ -238,6 +245,95 @@ def bench_batch(ctx, w, args):
     return sets[0][0], lo, hi, n, ms, tims, None, extra, {"mode": "batched", "sets": len(sets)}
 
 
+def bench_queries(ctx, w, args):
+    """tess_diagram_query_device over HBM-resident query positions; one step
+    is one call (count pass, scan, sizing, fill pass), results left in HBM."""
+    pts = tess.clustered(w["points"], w["seed"])
+    n = len(pts)
+    d = tess.Diagram(ctx, pts, np.zeros(3), np.ones(3), density=w["density"])
+    r = w["radius"]
+    qt = tess.DeviceArray(ctx, pts)
+    rt = tess.DeviceArray(ctx, np.array([r]))
+    # exact work counts of the walk (untimed): in-range grid cells, candidate
+    # points of those cells, kept points
+    sizes = {}
+    for kind in (tess.QUERY_CELLS, tess.QUERY_CELL_RADIUS, tess.QUERY_REAL_RADIUS):
+        res = d.query_device(kind, qt.ptr, n, rt.ptr, 1)
+        sizes[kind] = int(res.total)
+        res.close()
+    cells, cands, kept = sizes[tess.QUERY_CELLS], sizes[tess.QUERY_CELL_RADIUS], sizes[tess.QUERY_REAL_RADIUS]
+
+    def step():
+        res = d.query_device(tess.QUERY_REAL_RADIUS, qt.ptr, n, rt.ptr, 1)
+        t = res.timings()
+        res.close()
+        return t
+
+    ms, tims = timed(step, args.steps, args.warmup, ctx)
+    fill_ms = mean(tims, "fill_ms")
+    count_ms = mean(tims, "count_ms")
+    # algorithmic bytes of the fill pass: per query the position (24 B) and its
+    # two CSR offsets (16 B); per in-range grid cell its two delimiters (8 B);
+    # per candidate its sorted index and gathered position (28 B); per kept
+    # point its id (4 B)
+    fill_bytes = 40 * n + 8 * cells + 28 * cands + 4 * kept
+    achieved = fill_bytes / (fill_ms / 1e3) / 1e9
+    line = {
+        "metric": "radius queries/sec (find_neighbors_in_real_radius, f64, 1 MI355X)",
+        "value": n / (ms / 1e3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded xoshiro256**)",
+        "config": {"workload": w["desc"], "points": n, "queries": n, "radius": r, "cell_density": w["density"]},
+        "phases_ms": {"count": count_ms, "fill": fill_ms, "call": mean(tims, "total_ms")},
+        "work": {"grid_cells": cells, "candidates": cands, "kept": kept},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "fill pass k_query_box<2, true>, HIP events on the context stream",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "bytes_per_launch": fill_bytes,
+            "kernel_ms": fill_ms,
+            "traffic": None,
+        },
+    }
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath)).get(args.workload)
+        if rec:
+            # the per-query bytes count every query's own reads; neighbouring
+            # queries share grid cells, so the HBM bytes are fewer and the
+            # algorithmic rate may exceed the HBM peak
+            line["roofline"]["traffic"] = rec["bytes_per_launch"]
+            line["roofline"]["traffic_kernel"] = rec["kernel"]
+            line["roofline"]["hbm_gbs"] = rec["bytes_per_launch"] / (fill_ms / 1e3) / 1e9
+            line["roofline"]["hbm_frac"] = line["roofline"]["hbm_gbs"] / HBM_PEAK_GBS
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        m = min(args.cpu_sample, 200_000)
+        sel = np.random.default_rng(0).choice(n, m, replace=False)
+        t0 = time.perf_counter()
+        off, _ = orc.queries(tess.QUERY_REAL_RADIUS, pts, pts[sel], r, density=w["density"])
+        sec = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": m / sec, "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"{m} random queries of the same set (seed 0 choice), grid build included, {sec:.1f} s "
+                      f"on 1 thread",
+        }
+    d.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +357,9 @@ def main():
 
     w = WORKLOADS[args.workload]
     ctx = tess.Context(0)
+    if w["kind"] == "queries":
+        bench_queries(ctx, w, args)
+        return
     fn = {"clustered": bench_clustered, "uniform": bench_uniform, "batch": bench_batch}[w["kind"]]
     pts, lo, hi, n, ms, tims, cells, extra, cfg_extra = fn(ctx, w, args)
     value = n / (ms / 1e3)

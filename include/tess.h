@@ -212,14 +212,20 @@ enum {
 typedef struct tess_query tess_query;
 tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, size_t nq, const double* radius,
                                size_t nradius, int64_t target_group, tess_query** out);
+/* Same with device-resident positions and radii (read during the call). */
+tess_status tess_diagram_query_device(tess_diagram* d, int kind, const double* device_xyz, size_t nq,
+                                      const double* device_radius, size_t nradius, int64_t target_group,
+                                      tess_query** out);
 size_t tess_query_count(const tess_query* q);   /* number of queries */
 uint64_t tess_query_total(const tess_query* q); /* ids over all queries */
 /* offsets: nq + 1 entries; ids: tess_query_total entries (either may be NULL) */
 tess_status tess_query_copy(const tess_query* q, uint64_t* offsets, uint32_t* ids);
 /* Device pointers owned by q (valid until tess_query_destroy). */
 tess_status tess_query_device(const tess_query* q, const uint64_t** offsets, const uint32_t** ids);
-/* Device time of the query passes (count, scan, fill), ms. */
-double tess_query_ms(const tess_query* q);
+/* Device time, ms: [0] the count pass, [1] the fill pass, [2] the whole
+ * call from the count pass to the end of the fill pass (with the scan and
+ * the sizing of the id buffer between them). */
+tess_status tess_query_timings(const tess_query* q, double ms[3]);
 void tess_query_destroy(tess_query* q);
 
 /* ---- learned search-radius model (no counterpart in the reference) -----

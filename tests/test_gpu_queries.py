@@ -171,3 +171,65 @@ def test_million_points_properties(ctx):
         seg = ids[off[a]:off[a + 1]]
         assert len(seg) == n and np.array_equal(np.sort(seg), np.arange(n, dtype=np.uint32))
     d.close()
+
+
+def test_device_inputs(ctx):
+    """tess_diagram_query_device: positions and radii already in HBM give the
+    host-input results, and the results stay on the device."""
+    pts = tess.clustered(20000, 5)
+    d = _diagram(ctx, pts, 5.0)
+    q, r = _random_queries(pts, 6, 256)
+    qt = tess.DeviceArray(ctx, q)
+    for kind in KINDS:
+        rr = np.where(np.isfinite(r), np.sign(r) * r * r * 2.0, r) if kind == tess.QUERY_CLOUD else r
+        rt = tess.DeviceArray(ctx, rr)
+        res = d.query_device(kind, qt.ptr, len(q), rt.ptr, len(rr))
+        off, ids = res.copy()
+        t = res.timings()
+        assert t["count_ms"] > 0 and t["fill_ms"] > 0 and t["total_ms"] >= t["fill_ms"]
+        res.close()
+        ro, rid = orc.queries(kind, pts, q, rr, density=5.0)
+        assert np.array_equal(off, ro) and np.array_equal(ids, rid)
+        rt.close()
+    qt.close()
+    d.close()
+
+
+def test_device_resident_diagram(ctx):
+    """tess_diagram_create_device (points and groups already in HBM) gives
+    the cells and query results of the host-input diagram."""
+    pts = tess.clustered(30000, 12)
+    groups = np.random.default_rng(13).integers(0, 2, len(pts)).astype(np.int64)
+    lo, hi = np.zeros(3), np.ones(3)
+    ref = tess.Diagram(ctx, pts, lo, hi, groups=groups, density=5.0)
+    ref.compute()
+    want = ref.cells()
+    pd, gd = tess.DeviceArray(ctx, pts), tess.DeviceArray(ctx, groups)
+    d = tess.Diagram(ctx, pd, lo, hi, groups=gd, density=5.0)
+    d.compute()
+    got = d.cells()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    q = pts[:100]
+    a = d.query(tess.QUERY_CLOUD, q, 1e-4, target_group=1)
+    b = ref.query(tess.QUERY_CLOUD, q, 1e-4, target_group=1)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    d.close()
+    ref.close()
+    pd.close()
+    gd.close()
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_large_query_sets_in_grid_order(ctx, kind):
+    """Sets of 4096 queries and more run in the grid order of their positions;
+    the CSR is still in input order and bitwise the oracle's."""
+    pts = tess.clustered(20000, 31)
+    d = _diagram(ctx, pts, 5.0)
+    q, r = _random_queries(pts, 32, 6000)
+    if kind == tess.QUERY_CLOUD:
+        r = np.where(np.isfinite(r), np.sign(r) * r * r * 2.0, r)
+    off, ids = d.query(kind, q, r)
+    ro, rid = orc.queries(kind, pts, q, r, density=5.0)
+    assert np.array_equal(off, ro) and np.array_equal(ids, rid)
+    d.close()

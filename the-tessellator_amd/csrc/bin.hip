@@ -417,6 +417,22 @@ hipError_t launch_counting_sort(const double* x, const double* y, const double* 
   return hipGetLastError();
 }
 
+hipError_t launch_grid_order(const double* x, const double* y, const double* z, int n, const GridDev* g,
+                             int nbins, const SortBufs& b, hipStream_t s) {
+  hipError_t e;
+  if ((e = launch_fill_u32(b.cnt, (size_t)nbins, 0u, s)) != hipSuccess) return e;
+  if ((e = launch_fill_u32(b.misc, 4, 0u, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt);
+  const int nsb = blocks(nbins, 1024);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, b.cnt, nbins, b.delim, b.sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, b.sums, nsb, b.misc);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sums,
+                     b.misc, b.delim + nbins);
+  hipLaunchKernelGGL(k_copy_u32, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.cursor);
+  hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, b.bins, n, b.cursor, b.sorted);
+  return hipGetLastError();
+}
+
 hipError_t launch_scan_u32(const uint32_t* in, int m, uint32_t* out, uint32_t* sums, uint32_t* total,
                            hipStream_t s) {
   if (m <= 0) return hipSuccess;

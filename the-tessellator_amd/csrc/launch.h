@@ -22,6 +22,11 @@ struct SortBufs {
 };
 hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
                                 const GridDev* g, int nbins, const SortBufs& b, hipStream_t s);
+// The same binning without the in-cell ordering: b.sorted lists the n
+// positions grid cell by grid cell, in no set order inside a cell (only
+// bins, cnt, delim, cursor, sorted, sums and misc are used)
+hipError_t launch_grid_order(const double* x, const double* y, const double* z, int n, const GridDev* g,
+                             int nbins, const SortBufs& b, hipStream_t s);
 // exclusive scan of m u32 (sums: ceil(m / 1024) + 1 entries); *total = sum
 hipError_t launch_scan_u32(const uint32_t* in, int m, uint32_t* out, uint32_t* sums, uint32_t* total,
                            hipStream_t s);

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <limits>
 #include <memory>
+#include <vector>
 
 #include "launch.h"
 #include "runtime.h"
@@ -12,7 +13,7 @@ struct tess_query {
   tess_ctx* ctx = nullptr;
   size_t nq = 0;
   uint64_t total = 0;
-  double ms = 0;
+  double ms[3] = {0, 0, 0};  // count pass, fill pass, whole call
   DevBuf<uint64_t> offsets;
   DevBuf<uint32_t> ids;
 };
@@ -37,8 +38,11 @@ int cloud_table_extent(int64_t cpd, const GridDev& G, const double* r, size_t m)
   return L < full ? L : full;
 }
 
+// Query sets at least this large are taken in grid order.
+constexpr size_t kQueryOrderMin = 4096;
+
 struct Events {
-  hipEvent_t e[2] = {nullptr, nullptr};
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   Events() {
     for (auto& x : e) (void)hipEventCreate(&x);
   }
@@ -48,12 +52,8 @@ struct Events {
   }
 };
 
-}  // namespace
-
-extern "C" {
-
-tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, size_t nq, const double* radius,
-                               size_t nradius, int64_t target_group, tess_query** out) {
+tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, const double* radius,
+                       size_t nradius, int64_t target_group, bool device_inputs, tess_query** out) {
   if (!d || !out) return TESS_E_INVALID;
   *out = nullptr;
   tess_ctx* c = d->ctx;
@@ -83,21 +83,55 @@ tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, siz
   const GridDev& G = d->hgrid;
   const Table* t = nullptr;
   if (kind == TESS_QUERY_CLOUD) {
-    const int L = cloud_table_extent(d->cpd, G, radius, nradius);
+    // the table extent needs the radii on the host
+    std::vector<double> hr;
+    const double* r = radius;
+    if (device_inputs) {
+      hr.resize(nradius);
+      if ((st = c->hip(hipMemcpy(hr.data(), radius, 8 * nradius, hipMemcpyDeviceToHost), "download")) != TESS_OK)
+        return st;
+      r = hr.data();
+    }
+    const int L = cloud_table_extent(d->cpd, G, r, nradius);
     if (L > 511) return c->fail(TESS_E_INVALID, "radius spans more than 511 grid cells");
     if (!(t = c->table(d->cpd, G.sx, G.sy, G.sz, L, &st))) return st;
   }
   DevBuf<double> qd, rd;
   DevBuf<uint32_t> cnt;
   DevBuf<uint64_t> sums;
-  if (qd.alloc(3 * nq) != hipSuccess || rd.alloc(nradius) != hipSuccess || cnt.alloc(nq) != hipSuccess ||
-      sums.alloc((nq + 1023) / 1024 + 1) != hipSuccess)
+  if (cnt.alloc(nq) != hipSuccess || sums.alloc((nq + 1023) / 1024 + 1) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  if ((st = c->hip(hipMemcpyAsync(qd.p, xyz, 24 * nq, hipMemcpyHostToDevice, s), "upload")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(rd.p, radius, 8 * nradius, hipMemcpyHostToDevice, s), "upload")) != TESS_OK)
-    return st;
+  if (device_inputs) {
+    qd.view(const_cast<double*>(xyz), 3 * nq);
+    rd.view(const_cast<double*>(radius), nradius);
+  } else {
+    if (qd.alloc(3 * nq) != hipSuccess || rd.alloc(nradius) != hipSuccess)
+      return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if ((st = c->hip(hipMemcpyAsync(qd.p, xyz, 24 * nq, hipMemcpyHostToDevice, s), "upload")) != TESS_OK ||
+        (st = c->hip(hipMemcpyAsync(rd.p, radius, 8 * nradius, hipMemcpyHostToDevice, s), "upload")) != TESS_OK)
+      return st;
+  }
+  // Large sets: take the queries in grid order of their positions (the
+  // results do not depend on the order the wavefronts take them in).
+  DevBuf<double> ox, oy, oz;
+  DevBuf<uint32_t> obins, ocnt, odelim, ocursor, oorder, osums, omisc;
+  const bool ordered = nq >= kQueryOrderMin;
+  if (ordered) {
+    const size_t nb = (size_t)d->nbins;
+    if (ox.alloc(nq) != hipSuccess || oy.alloc(nq) != hipSuccess || oz.alloc(nq) != hipSuccess ||
+        obins.alloc(nq) != hipSuccess || oorder.alloc(nq) != hipSuccess || ocnt.alloc(nb) != hipSuccess ||
+        odelim.alloc(nb + 1) != hipSuccess || ocursor.alloc(nb) != hipSuccess ||
+        osums.alloc((nb + 1023) / 1024 + 1) != hipSuccess || omisc.alloc(4) != hipSuccess)
+      return c->fail(TESS_E_NOMEM, "device allocation failed");
+    SortBufs b{obins.p, ocnt.p, odelim.p, ocursor.p, oorder.p, nullptr, osums.p, nullptr, omisc.p};
+    if ((st = c->hip(launch_soa(qd.p, (int)nq, ox.p, oy.p, oz.p, s), "query order")) != TESS_OK ||
+        (st = c->hip(launch_grid_order(ox.p, oy.p, oz.p, (int)nq, d->grid.p, d->nbins, b, s), "query order")) !=
+            TESS_OK)
+      return st;
+  }
   QueryArgs a{};
   a.g = G;
+  a.order = ordered ? oorder.p : nullptr;
   a.sorted = d->sorted.p;
   a.delim = d->delim.p;
   a.sx = d->sx.p;
@@ -117,6 +151,7 @@ tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, siz
   Events ev;
   if ((st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
       (st = c->hip(launch_query_count(a, s), "query count")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
       (st = c->hip(launch_face_offsets(cnt.p, (int)nq, q->offsets.p, sums.p, s), "query scan")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&q->total, q->offsets.p + nq, 8, hipMemcpyDeviceToHost, s), "download")) !=
           TESS_OK ||
@@ -125,20 +160,42 @@ tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, siz
   if (q->ids.alloc(q->total) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
   a.offsets = q->offsets.p;
   a.ids = q->ids.p;
-  float ms = 0;
-  if ((st = c->hip(launch_query_fill(a, s), "query fill")) != TESS_OK ||
-      (st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
+  float ms[3] = {0, 0, 0};
+  if ((st = c->hip(hipEventRecord(ev.e[2], s), "event")) != TESS_OK ||
+      (st = c->hip(launch_query_fill(a, s), "query fill")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev.e[3], s), "event")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "query fill")) != TESS_OK ||
-      (st = c->hip(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "event")) != TESS_OK)
+      (st = c->hip(hipEventElapsedTime(&ms[0], ev.e[0], ev.e[1]), "event")) != TESS_OK ||
+      (st = c->hip(hipEventElapsedTime(&ms[1], ev.e[2], ev.e[3]), "event")) != TESS_OK ||
+      (st = c->hip(hipEventElapsedTime(&ms[2], ev.e[0], ev.e[3]), "event")) != TESS_OK)
     return st;
-  q->ms = ms;
+  for (int k = 0; k < 3; ++k) q->ms[k] = ms[k];
   *out = q.release();
   return TESS_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+tess_status tess_diagram_query(tess_diagram* d, int kind, const double* xyz, size_t nq, const double* radius,
+                               size_t nradius, int64_t target_group, tess_query** out) {
+  return query_impl(d, kind, xyz, nq, radius, nradius, target_group, false, out);
+}
+
+tess_status tess_diagram_query_device(tess_diagram* d, int kind, const double* device_xyz, size_t nq,
+                                      const double* device_radius, size_t nradius, int64_t target_group,
+                                      tess_query** out) {
+  return query_impl(d, kind, device_xyz, nq, device_radius, nradius, target_group, true, out);
+}
+
 size_t tess_query_count(const tess_query* q) { return q ? q->nq : 0; }
 uint64_t tess_query_total(const tess_query* q) { return q ? q->total : 0; }
-double tess_query_ms(const tess_query* q) { return q ? q->ms : 0.0; }
+tess_status tess_query_timings(const tess_query* q, double ms[3]) {
+  if (!q || !ms) return TESS_E_INVALID;
+  for (int k = 0; k < 3; ++k) ms[k] = q->ms[k];
+  return TESS_OK;
+}
 
 tess_status tess_query_copy(const tess_query* q, uint64_t* offsets, uint32_t* ids) {
   if (!q) return TESS_E_INVALID;

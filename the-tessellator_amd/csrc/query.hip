@@ -6,11 +6,13 @@
 // second writes the ids in the reference's order.  Box queries walk the index
 // box of the query in (i, j, k) loop order, 64 grid cells per step; the
 // neighbour cloud walks the shell table in canonical order, 64 entries per
-// step, until the first key above the radius.  Each lane owns one grid cell
-// of the step and lists that cell's points in sorted order, so a wave-wide
-// exclusive scan of the per-lane counts gives every lane its output position.
-// The walk is bound by the reads of delimiters, sorted indices and (for the
-// real radius) gathered coordinates: integer and byte work on HBM.
+// step, until the first key above the radius.  The points of a step's cells
+// are then walked one per lane, so loads and stores of neighbouring lanes
+// are neighbouring addresses and a dense cell does not hold up the wave.
+// Large query sets are taken in grid order of their positions, so the
+// wavefronts running together read the same grid cells.  The walk is bound
+// by the reads of delimiters, sorted indices and (for the real radius)
+// gathered coordinates: integer and byte work on HBM.
 #include "launch.h"
 #include "tess_internal.h"
 
@@ -48,34 +50,60 @@ __device__ __forceinline__ uint32_t wave_excl(uint32_t v, int lane, uint32_t* to
   return incl - v;
 }
 
-// Points of grid cell b that the query keeps: all (cell radius, cloud
-// without group), the ones within the radius (real radius), or the ones of
-// the target group (cloud).  Written from ids[pos] on when ids is set.
-template <int KIND>
-__device__ __forceinline__ uint32_t cell_points(const QueryArgs& a, uint32_t b, double qx, double qy, double qz,
-                                                double r2, uint32_t* ids, uint64_t pos, uint64_t end) {
-  const uint32_t s0 = a.delim[b], s1 = a.delim[b + 1];
-  if (KIND == 1 || (KIND == 3 && !a.groups)) {
-    if (ids)
-      for (uint32_t s = s0; s < s1; ++s)
-        if (pos + (s - s0) < end) ids[pos + (s - s0)] = a.sorted[s];
-    return s1 - s0;
+// The points of the step's 64 grid cells, in cell order and, inside a cell,
+// in grid (sorted-index) order: lane l's cell holds len points from grid
+// position s0 on (len 0: no cell).  The concatenation is walked 64 points at
+// a time, one point per lane; a lane finds its cell by binary search over the
+// cells' exclusive starts, read from their lanes.  MODE 0 keeps every point
+// (only the count is needed in the count pass), MODE 1 the points within the
+// radius (celery.rs:700-702, :843-851), MODE 2 the points of the target group
+// (interface.rs:358-361); kept points are written in walk order (ballot rank).
+template <int MODE, bool FILL>
+__device__ __forceinline__ void emit_points(const QueryArgs& a, uint32_t s0, uint32_t len, int lane, double qx,
+                                            double qy, double qz, double r2, uint64_t& cursor, uint64_t end) {
+  uint32_t total;
+  const uint32_t start = wave_excl(len, lane, &total);
+  if (MODE == 0 && !FILL) {
+    cursor += total;
+    return;
   }
-  uint32_t c = 0;
-  for (uint32_t s = s0; s < s1; ++s) {
-    bool keep;
-    if (KIND == 2) {
-      const double dx = qx - a.sx[s], dy = qy - a.sy[s], dz = qz - a.sz[s];  // celery.rs:700-702
-      keep = (dx * dx + dy * dy) + dz * dz <= r2;
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t p = base + (uint32_t)lane;
+    int c = 0;  // largest cell whose start is <= p (empty cells share the next start)
+#pragma unroll
+    for (int step = 32; step; step >>= 1) {
+      const uint32_t v = __shfl(start, c + step, 64);
+      if (v <= p) c += step;
+    }
+    const uint32_t cs0 = __shfl(s0, c, 64), cstart = __shfl(start, c, 64);
+    const bool valid = p < total;
+    const uint32_t s = cs0 + (p - cstart);
+    uint32_t id = 0;
+    bool keep = false;
+    if (valid) {
+      id = a.sorted[s];
+      if (MODE == 0) {
+        keep = true;
+      } else if (MODE == 1) {
+        const double dx = qx - a.sx[s], dy = qy - a.sy[s], dz = qz - a.sz[s];
+        keep = (dx * dx + dy * dy) + dz * dz <= r2;
+      } else {
+        keep = a.groups[id] == a.target_group;
+      }
+    }
+    if (MODE == 0) {
+      if (valid && cursor + p < end) a.ids[cursor + p] = id;
     } else {
-      keep = a.groups[a.sorted[s]] == a.target_group;  // interface.rs:358-361
-    }
-    if (keep) {
-      if (ids && pos + c < end) ids[pos + c] = a.sorted[s];
-      ++c;
+      const uint64_t m = __ballot(keep);
+      if (FILL && keep) {
+        const uint64_t pos = cursor + (uint64_t)__popcll(m & below);
+        if (pos < end) a.ids[pos] = id;
+      }
+      cursor += (uint64_t)__popcll(m);
     }
   }
-  return c;
+  if (MODE == 0) cursor += total;
 }
 
 // find_cells_in_radius (KIND 0) and the two neighbour searches built on it
@@ -83,8 +111,9 @@ __device__ __forceinline__ uint32_t cell_points(const QueryArgs& a, uint32_t b, 
 template <int KIND, bool FILL>
 __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
   const int lane = threadIdx.x & 63;
-  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (qi >= a.nq) return;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= a.nq) return;
+  const int qi = a.order ? (int)a.order[w] : w;
   const GridDev& G = a.g;
   const int cpd = G.cpd;
   const double qx = a.q[3 * (size_t)qi], qy = a.q[3 * (size_t)qi + 1], qz = a.q[3 * (size_t)qi + 2];
@@ -104,6 +133,7 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
   const uint64_t total = ni > 0 ? (uint64_t)ni * njk : 0;
   uint64_t cursor = FILL ? a.offsets[qi] : 0;
   const uint64_t end = FILL ? a.offsets[qi + 1] : 0;
+  const uint64_t below = (1ull << lane) - 1ull;
   for (uint64_t base = 0; base < total; base += 64) {
     const uint64_t idx = base + (uint64_t)lane;
     bool in = false;
@@ -117,19 +147,21 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
       in = (gx * gx + gy * gy) + gz * gz <= r2;
       b = ((uint32_t)i * (uint32_t)cpd + (uint32_t)j) * (uint32_t)cpd + (uint32_t)k;  // celery.rs:317-325
     }
-    uint32_t c = 0;
-    if (in) c = KIND == 0 ? 1u : cell_points<KIND>(a, b, qx, qy, qz, r2, nullptr, 0, 0);
-    uint32_t tot;
-    const uint32_t ex = wave_excl(c, lane, &tot);
-    if (FILL && c) {
-      const uint64_t pos = cursor + ex;
-      if (KIND == 0) {
+    if (KIND == 0) {
+      const uint64_t m = __ballot(in);
+      if (FILL && in) {
+        const uint64_t pos = cursor + (uint64_t)__popcll(m & below);
         if (pos < end) a.ids[pos] = b;
-      } else {
-        cell_points<KIND>(a, b, qx, qy, qz, r2, a.ids, pos, end);
       }
+      cursor += (uint64_t)__popcll(m);
+    } else {
+      uint32_t s0 = 0, len = 0;
+      if (in) {
+        s0 = a.delim[b];
+        len = a.delim[b + 1] - s0;
+      }
+      emit_points<KIND == 1 ? 0 : 1, FILL>(a, s0, len, lane, qx, qy, qz, r2, cursor, end);
     }
-    cursor += tot;
   }
   if (!FILL && lane == 0) a.count[qi] = (uint32_t)cursor;
 }
@@ -140,8 +172,9 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_query_cloud(QueryArgs a) {
   const int lane = threadIdx.x & 63;
-  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (qi >= a.nq) return;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= a.nq) return;
+  const int qi = a.order ? (int)a.order[w] : w;
   const GridDev& G = a.g;
   const int cpd = G.cpd;
   const double qx = a.q[3 * (size_t)qi], qy = a.q[3 * (size_t)qi + 1], qz = a.q[3 * (size_t)qi + 2];
@@ -157,20 +190,21 @@ __global__ __launch_bounds__(256) void k_query_cloud(QueryArgs a) {
     const double key = have ? a.key[t] : 0.0;
     const bool beyond = have && key > r;  // celery.rs:1036: the loop ends here
     const bool take = have && !beyond;    // keys ascend: the taken lanes are a prefix
-    uint32_t c = 0, b = 0;
+    uint32_t s0 = 0, len = 0;
     if (take) {
       int ox, oy, oz;
       unpack_off(a.off[t], ox, oy, oz);
       const int bx = hx + ox, by = hy + oy, bz = hz + oz;
       if (bx >= 0 && bx < cpd && by >= 0 && by < cpd && bz >= 0 && bz < cpd) {
-        b = ((uint32_t)bx * (uint32_t)cpd + (uint32_t)by) * (uint32_t)cpd + (uint32_t)bz;
-        c = cell_points<3>(a, b, qx, qy, qz, 0.0, nullptr, 0, 0);
+        const uint32_t b = ((uint32_t)bx * (uint32_t)cpd + (uint32_t)by) * (uint32_t)cpd + (uint32_t)bz;
+        s0 = a.delim[b];
+        len = a.delim[b + 1] - s0;
       }
     }
-    uint32_t tot;
-    const uint32_t ex = wave_excl(c, lane, &tot);
-    if (FILL && c) cell_points<3>(a, b, qx, qy, qz, 0.0, a.ids, cursor + ex, end);
-    cursor += tot;
+    if (a.groups)
+      emit_points<2, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end);
+    else
+      emit_points<0, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end);
     if (__ballot(beyond)) break;
   }
   if (!FILL && lane == 0) a.count[qi] = (uint32_t)cursor;

@@ -110,6 +110,7 @@ struct QueryArgs {
   const int64_t* groups;   // by point id; null: no group filter
   int64_t target_group;
   const double* q;         // query positions, AoS [3 nq]
+  const uint32_t* order;   // nullable: wavefront w takes query order[w]
   const double* radius;    // radius of query i: radius[i * rstride]
   int rstride;
   int nq;

@@ -37,8 +37,8 @@ EXPORTS = [
     "tess_diagram_compute_scheduled", "tess_diagram_features", "tess_mlp_forward", "tess_mlp_gradient",
     "tess_mfma_probe", "tess_batch_create", "tess_batch_destroy", "tess_batch_sets", "tess_batch_set_config",
     "tess_batch_get_config", "tess_batch_compute", "tess_batch_tune", "tess_batch_total_faces",
-    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith", "tess_diagram_query", "tess_query_count",
-    "tess_query_total", "tess_query_copy", "tess_query_device", "tess_query_ms", "tess_query_destroy",
+    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith", "tess_diagram_query", "tess_diagram_query_device", "tess_query_count",
+    "tess_query_total", "tess_query_copy", "tess_query_device", "tess_query_timings", "tess_query_destroy",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 MLP_IN, MLP_PARAMS = 32, 7312
@@ -147,11 +147,12 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_batch_timings": (C.c_int, [vp, C.POINTER(Timings)]),
         "tess_diag_arith": (C.c_int, [vp, dp, dp, sz, dp, dp, dp]),
         "tess_diagram_query": (C.c_int, [vp, C.c_int, dp, sz, dp, sz, C.c_int64, C.POINTER(vp)]),
+        "tess_diagram_query_device": (C.c_int, [vp, C.c_int, vp, sz, vp, sz, C.c_int64, C.POINTER(vp)]),
         "tess_query_count": (sz, [vp]),
         "tess_query_total": (C.c_uint64, [vp]),
         "tess_query_copy": (C.c_int, [vp, dp, dp]),
         "tess_query_device": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
-        "tess_query_ms": (C.c_double, [vp]),
+        "tess_query_timings": (C.c_int, [vp, dp]),
         "tess_query_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
@@ -167,6 +168,57 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime libtess.so is linked against (already loaded with it)."""
+    global _hip_rt
+    if _hip_rt is None:
+        load_library()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipSetDevice.argtypes = [C.c_int]
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        for f in (h.hipSetDevice, h.hipMalloc, h.hipFree, h.hipMemcpy):
+            f.restype = C.c_int
+        _hip_rt = h
+    return _hip_rt
+
+
+class DeviceArray:
+    """A copy of a numpy array in device memory, for the entry points that
+    take device-resident inputs (tess_diagram_create_device,
+    tess_diagram_query_device).  Allocated with the HIP runtime libtess.so
+    uses: a second HIP runtime in the process (such as the one bundled with
+    a PyTorch wheel) would not share its device state."""
+
+    def __init__(self, ctx, a):
+        a = np.ascontiguousarray(a)
+        self.shape, self.dtype, self.nbytes = a.shape, a.dtype, a.nbytes
+        hip = _hip()
+        hip.hipSetDevice(ctx.device)
+        p = C.c_void_p()
+        if hip.hipMalloc(C.byref(p), max(1, a.nbytes)) != 0:
+            raise TessError(2, "hipMalloc failed")
+        self.ptr = p
+        if a.nbytes and hip.hipMemcpy(p, _ptr(a), a.nbytes, 1) != 0:  # host to device
+            self.close()
+            raise TessError(3, "hipMemcpy failed")
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP device + stream (tess_ctx)."""
 
@@ -177,6 +229,7 @@ class Context:
         if st != TESS_OK:
             raise TessError(st, f"cannot open HIP device {device}")
         self.h = h
+        self.device = device
         self._diagrams = weakref.WeakSet()  # destroyed before the context
 
     def check(self, st):
@@ -274,19 +327,27 @@ class Diagram:
     def __init__(self, ctx: Context, xyz, lo, hi, groups=None, density=1.25, mode=MODE_PLAIN,
                  vertex_capacity=0, collect_stats=False, xcd_chunk=0):
         self.ctx, self.lib = ctx, ctx.lib
-        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
-        self.n = xyz.shape[0]
-        self.xyz = xyz
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
-        g = None if groups is None else np.ascontiguousarray(groups, np.int64)
         o = Opts()
         self.lib.tess_opts_default(C.byref(o))
         o.cell_density, o.mode, o.vertex_capacity, o.collect_stats = density, mode, vertex_capacity, int(collect_stats)
         o.xcd_chunk = xcd_chunk
         h = C.c_void_p()
-        ctx.check(self.lib.tess_diagram_create(ctx.h, _ptr(xyz), self.n, _ptr(lo), _ptr(hi), _ptr(g),
-                                               C.byref(o), C.byref(h)))
+        if isinstance(xyz, DeviceArray):  # device-resident AoS points (and groups)
+            assert xyz.dtype == np.float64 and xyz.nbytes % 24 == 0
+            self.n = xyz.nbytes // 24
+            self.xyz = None
+            gp = None if groups is None else groups.ptr
+            ctx.check(self.lib.tess_diagram_create_device(ctx.h, xyz.ptr, self.n, _ptr(lo), _ptr(hi), gp,
+                                                          C.byref(o), C.byref(h)))
+        else:
+            xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+            self.n = xyz.shape[0]
+            self.xyz = xyz
+            g = None if groups is None else np.ascontiguousarray(groups, np.int64)
+            ctx.check(self.lib.tess_diagram_create(ctx.h, _ptr(xyz), self.n, _ptr(lo), _ptr(hi), _ptr(g),
+                                                   C.byref(o), C.byref(h)))
         self.h = h
         ctx._diagrams.add(self)
 
@@ -368,20 +429,29 @@ class Diagram:
     def query(self, kind, positions, radius, target_group=None):
         """Radius queries (tess_diagram_query): CSR offsets [nq + 1] and ids;
         the ids of query i are ids[offsets[i]:offsets[i + 1]].  radius: one
-        value or one per query.  The device time is kept in last_query_ms."""
+        value or one per query.  Device times (count pass, fill pass, whole
+        call) are kept in last_query_ms."""
         q = np.ascontiguousarray(np.asarray(positions, np.float64).reshape(-1, 3))
         r = np.ascontiguousarray(np.atleast_1d(np.asarray(radius, np.float64)))
         g = -1 if target_group is None else int(target_group)
         h = C.c_void_p()
         self.ctx.check(self.lib.tess_diagram_query(self.h, kind, _ptr(q), len(q), _ptr(r), r.size, g, C.byref(h)))
+        res = QueryResult(self.ctx, h)
         try:
-            off = np.empty(len(q) + 1, np.uint64)
-            ids = np.empty(self.lib.tess_query_total(h), np.uint32)
-            self.ctx.check(self.lib.tess_query_copy(h, _ptr(off), _ptr(ids)))
-            self.last_query_ms = self.lib.tess_query_ms(h)
+            off, ids = res.copy()
+            self.last_query_ms = res.timings()
         finally:
-            self.lib.tess_query_destroy(h)
+            res.close()
         return off, ids
+
+    def query_device(self, kind, xyz_ptr, nq, radius_ptr, nradius, target_group=None):
+        """Radius queries over device-resident positions (AoS f64) and radii;
+        the results stay on the device in the returned QueryResult."""
+        g = -1 if target_group is None else int(target_group)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.tess_diagram_query_device(self.h, kind, xyz_ptr, nq, radius_ptr, nradius, g,
+                                                          C.byref(h)))
+        return QueryResult(self.ctx, h)
 
     def find_cells_in_radius(self, x, y, z, radius):
         """Celery::find_cells_in_radius (celery.rs:753-797): grid cell ids."""
@@ -397,7 +467,7 @@ class Diagram:
 
     def neighbor_cloud(self, index, radius, target_group=None):
         """Cell::compute_neighbor_cloud (interface.rs:348-365) of the cell of
-        point `index` (its position is the point's)."""
+        point `index` (its position is the point's; host-created diagrams)."""
         return self.query(QUERY_CLOUD, self.xyz[index], radius, target_group)[1]
 
     def get_cell_at_index(self, index, search_radius=None, target_group=None):
@@ -407,6 +477,37 @@ class Diagram:
         g = -1 if target_group is None else int(target_group)
         self.ctx.check(self.lib.tess_cell_at_index(self.h, index, r, g, C.byref(h)))
         return Cell(self.lib, h)
+
+
+class QueryResult:
+    """CSR results of one tess_diagram_query call, held on the device."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.lib, self.h = ctx, ctx.lib, h
+        self.nq = self.lib.tess_query_count(h)
+        self.total = self.lib.tess_query_total(h)
+
+    def copy(self):
+        off = np.empty(self.nq + 1, np.uint64)
+        ids = np.empty(self.total, np.uint32)
+        self.ctx.check(self.lib.tess_query_copy(self.h, _ptr(off), _ptr(ids)))
+        return off, ids
+
+    def timings(self):
+        ms = np.zeros(3)
+        self.ctx.check(self.lib.tess_query_timings(self.h, _ptr(ms)))
+        return dict(count_ms=float(ms[0]), fill_ms=float(ms[1]), total_ms=float(ms[2]))
+
+    def close(self):
+        if self.h:
+            self.lib.tess_query_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Model:
