@@ -178,6 +178,28 @@ size_t tess_cell_face_vertices(const tess_cell* c, size_t face, double* xyz, siz
 /* Cell::compute_vertices (interface.rs:368-370), site-relative. */
 size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
 
+/* ---- cells of outside sites (interface.rs:211-232) ---------------------
+ * Diagram::get_cell_at_particle + Cell::compute_voronoi_cell for m sites
+ * that are not points of the diagram (xyz: host AoS): each cell is the
+ * container box clipped by the bisectors between the site and the diagram's
+ * points (all of them, or those of target_group >= 0; no point is excluded
+ * as the cell's own).  search_radius: NaN = None, else the
+ * expand_all_in_radius cut-off of tess_cell_at_index.  Sites must lie in the
+ * container box (TESS_E_INVALID otherwise).  Results are laid out as
+ * tess_diagram_copy_cells, indexed by site. */
+typedef struct tess_site_cells tess_site_cells;
+tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, size_t m, double search_radius,
+                                            int64_t target_group, tess_site_cells** out);
+size_t tess_site_cells_count(const tess_site_cells* p);
+tess_status tess_site_cells_total_faces(const tess_site_cells* p, uint64_t* out);
+tess_status tess_site_cells_copy(const tess_site_cells* p, double* volume, uint32_t* face_count,
+                                 uint64_t* face_offset, int32_t* neighbor, double* face_area);
+/* Per-site counters (the diagram must have been created with collect_stats). */
+tess_status tess_site_cells_copy_stats(const tess_site_cells* p, tess_cell_stats* stats);
+/* ms[0]: cell passes, ms[1]: face offsets + compaction; reruns at capacity 256. */
+tess_status tess_site_cells_timings(const tess_site_cells* p, double ms[2], uint32_t* capacity_reruns);
+void tess_site_cells_destroy(tess_site_cells* p);
+
 /* ---- radius queries (celery.rs:700-855, interface.rs:348-365) ---------
  * Many queries of one kind over a diagram's grid (the grid is built first
  * if no compute has built it), with results as CSR: the ids of query i are

@@ -944,7 +944,9 @@ struct FaceCell {  // per-face polygon clip
   }
 };
 
-// compute_voronoi_cell (interface.rs:257-313) for point `i`.
+// compute_voronoi_cell (interface.rs:257-313) for point `i`, or for the
+// outside site *ext (Diagram::get_cell_at_particle, interface.rs:211-232:
+// no point is excluded as the cell's own).
 struct Filter {
   double max_radius = std::numeric_limits<double>::quiet_NaN();  // expand_all_in_radius cut-off
   const int64_t* groups = nullptr;                               // interface.rs:280-297
@@ -955,8 +957,8 @@ template <class Cell>
 void compute_cell(Grid& G, const double* x, const double* y, const double* z, size_t i,
                   const double lo[3], const double hi[3], int mode, Cell& C,
                   std::vector<FaceOut>& faces, double* volume, orc_cell_stats* st, bool check,
-                  const Filter& flt) {
-  const V3 site = {x[i], y[i], z[i]};
+                  const Filter& flt, const V3* ext = nullptr) {
+  const V3 site = ext ? *ext : V3{x[i], y[i], z[i]};
   C.init(lo, hi, site);
   uint64_t flops = 24 + 40;
   double r2 = C.r2();
@@ -1008,7 +1010,7 @@ void compute_cell(Grid& G, const double* x, const double* y, const double* z, si
     uint64_t b = (uint64_t)((bx * cpd + by) * cpd + bz);
     for (uint64_t k = G.delim[b]; k < G.delim[b + 1]; ++k) {
       uint32_t j = G.sorted[k];
-      if (j == i) continue;  // self (interface.rs:299-305, by internal index)
+      if (!ext && j == i) continue;  // self (interface.rs:299-305, by internal index)
       if (flt.groups && flt.target >= 0 && flt.groups[j] != flt.target) continue;
       s.streamed++;
       V3 p = {x[j] - site.x, y[j] - site.y, z[j] - site.z};
@@ -1292,6 +1294,62 @@ orc_result* orc_compute_query(const double* x, const double* y, const double* z,
         compute_cell(G, x, y, z, i, lo, hi, mode, W, faces, &R->volume[c], &R->stats[c], check, flt);
       else
         compute_cell(G, x, y, z, i, lo, hi, mode, F, faces, &R->volume[c], &R->stats[c], check, flt);
+      per[c] = faces;
+      R->fcount[c] = (uint32_t)faces.size();
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  R->seconds = std::chrono::duration<double>(t1 - t0).count();
+  R->foff.resize(ncells + 1);
+  R->foff[0] = 0;
+  for (size_t c = 0; c < ncells; ++c) R->foff[c + 1] = R->foff[c] + R->fcount[c];
+  R->nbr.resize(R->foff[ncells]);
+  R->area.resize(R->foff[ncells]);
+  for (size_t c = 0; c < ncells; ++c)
+    for (size_t k = 0; k < per[c].size(); ++k) {
+      R->nbr[R->foff[c] + k] = (int32_t)per[c][k].tag;
+      R->area[R->foff[c] + k] = per[c][k].area;
+    }
+  return R;
+}
+
+orc_result* orc_compute_at_points(const double* x, const double* y, const double* z, size_t n,
+                                  const double lo[3], const double hi[3], double density, int mode, int clip,
+                                  const double* sites, size_t nsites, int nthreads, double max_radius,
+                                  const int64_t* groups, int64_t target_group) {
+  Filter flt;
+  flt.max_radius = max_radius;
+  flt.groups = groups;
+  flt.target = target_group;
+  orc_result* R = new orc_result;
+  if (n == 0) return R;
+  auto t0 = std::chrono::steady_clock::now();
+  Grid G;
+  grid_setup(G, x, y, z, n, density, mode == ORC_MODE_ALL_POINTS);
+  const size_t ncells = nsites;
+  R->ncells = ncells;
+  R->volume.resize(ncells);
+  R->fcount.resize(ncells);
+  R->stats.resize(ncells);
+  std::vector<std::vector<FaceOut>> per(ncells);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    WalkCell W;
+    FaceCell F;
+    std::vector<FaceOut> faces;
+    const bool check = (clip & ORC_CHECK) != 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 256)
+#endif
+    for (size_t c = 0; c < ncells; ++c) {
+      const V3 site = {sites[3 * c], sites[3 * c + 1], sites[3 * c + 2]};
+      if ((clip & 1) == ORC_CLIP_WALK)
+        compute_cell(G, x, y, z, 0, lo, hi, mode, W, faces, &R->volume[c], &R->stats[c], check, flt, &site);
+      else
+        compute_cell(G, x, y, z, 0, lo, hi, mode, F, faces, &R->volume[c], &R->stats[c], check, flt, &site);
       per[c] = faces;
       R->fcount[c] = (uint32_t)faces.size();
     }

@@ -146,6 +146,13 @@ orc_result* orc_compute(const double* x, const double* y, const double* z, size_
 /* Same with the reference's query options: max_radius (NaN = None) stops the
  * stream at the first shell key above it (celery.rs:1023-1075) and
  * target_group >= 0 keeps only points of that group (interface.rs:280-297). */
+/* Cells of sites that are not points of the set (Diagram::get_cell_at_particle,
+ * interface.rs:211-232, + compute_voronoi_cell): nsites AoS positions; no
+ * point is excluded as the cell's own. */
+orc_result* orc_compute_at_points(const double* x, const double* y, const double* z, size_t n,
+                                  const double lo[3], const double hi[3], double density, int mode, int clip,
+                                  const double* sites, size_t nsites, int nthreads, double max_radius,
+                                  const int64_t* groups, int64_t target_group);
 orc_result* orc_compute_query(const double* x, const double* y, const double* z, size_t n,
                               const double lo[3], const double hi[3], double density, int mode, int clip,
                               const uint32_t* cells, size_t ncells, int nthreads, double max_radius,

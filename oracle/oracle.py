@@ -83,6 +83,8 @@ def lib():
             "orc_compute": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int]),
             "orc_compute_query": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int,
                                        C.c_double, dp, C.c_int64]),
+            "orc_compute_at_points": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int,
+                                           C.c_double, dp, C.c_int64]),
             "orc_result_cells": (sz, [vp]),
             "orc_result_total_faces": (C.c_uint64, [vp]),
             "orc_result_copy": (None, [vp, dp, dp, dp, dp, dp]),
@@ -291,6 +293,24 @@ def compute(xyz, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, cell
     r = lib().orc_compute_query(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(cl), nc, threads,
                                 float("nan") if max_radius is None else float(max_radius), _p(g),
                                 -1 if target_group is None else int(target_group))
+    return _result(r, nc)
+
+
+def compute_at_points(xyz, sites, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, threads=0,
+                      max_radius=None, groups=None, target_group=None):
+    """Cells of sites that are not points of the set (get_cell_at_particle), same dict as compute."""
+    x, y, z = soa(xyz)
+    n = len(x)
+    lo, hi = _v(lo), _v(hi)
+    st = np.ascontiguousarray(np.asarray(sites, np.float64).reshape(-1, 3))
+    g = None if groups is None else np.ascontiguousarray(groups, np.int64)
+    r = lib().orc_compute_at_points(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(st), len(st),
+                                    threads, float("nan") if max_radius is None else float(max_radius), _p(g),
+                                    -1 if target_group is None else int(target_group))
+    return _result(r, len(st))
+
+
+def _result(r, nc):
     try:
         f = lib().orc_result_total_faces(r)
         vol, fc = np.empty(nc), np.empty(nc, np.uint32)

@@ -703,8 +703,13 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   using L = Lds<VPL, PPL, FPL>;
   constexpr int FC = L::FC;
   CellState<VPL, PPL, FPL> C;
-  const uint32_t self = A.sorted[gpos];
-  const double sx = A.spx[gpos], sy = A.spy[gpos], sz = A.spz[gpos];
+  // a site of the set excludes itself by index (interface.rs:299-305); an
+  // outside site (interface.rs:211-232) excludes nothing
+  const bool ext = A.ext != nullptr;
+  const uint32_t self = ext ? 0xffffffffu : A.sorted[gpos];
+  const double sx = ext ? A.ext[3 * gpos] : A.spx[gpos];
+  const double sy = ext ? A.ext[3 * gpos + 1] : A.spy[gpos];
+  const double sz = ext ? A.ext[3 * gpos + 2] : A.spz[gpos];
   const GridDev G = *A.grid;
   const int cpd = G.cpd;
   const int ix = axis_index(sx, G.xmin, G.xmax, G.ix, cpd);
@@ -1013,7 +1018,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       // streamed: every candidate of the reached entries but the site itself
       const int rc = (ebase + lane < e_stop && lane < lim) ? cnt : 0;
       const int sc = rl_i(wave_incl_scan(rc, lane), 63);
-      streamed += (uint32_t)(sc - (ebase == 0 ? 1 : 0));
+      streamed += (uint32_t)(sc - (ebase == 0 && !ext ? 1 : 0));
     }
     wave_sync();
     if (stop_here || done) {
@@ -1024,7 +1029,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   flops += 8ull * streamed;
 
   // ---- outputs
-  const uint32_t oi = self;
+  const uint32_t oi = ext ? (uint32_t)gpos : self;
   int nfaces = popc_multi(C.FA);
   const bool ok = (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0;
   if (ok) {

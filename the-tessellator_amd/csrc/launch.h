@@ -68,6 +68,7 @@ hipError_t launch_batch_costs(const uint64_t* items, uint64_t total, const uint6
 hipError_t launch_batch_check(const uint32_t* delim, const uint64_t* dend, const uint64_t* base, int nsets,
                               uint32_t* bad, hipStream_t s);
 // status[o] |= flag (and stats[o].status when stats is not null), o = sorted[list[i]]
+// (o = list[i] when sorted is null)
 hipError_t launch_mark_status(const uint32_t* list, int m, const uint32_t* sorted, uint32_t* status,
                               tess_cell_stats* stats, uint32_t flag, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t m, uint32_t v, hipStream_t s);

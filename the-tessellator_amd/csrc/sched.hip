@@ -95,7 +95,7 @@ __global__ void k_mark_status(const uint32_t* __restrict__ list, int m, const ui
                               uint32_t* __restrict__ status, tess_cell_stats* __restrict__ stats, uint32_t flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const uint32_t o = sorted[list[i]];
+  const uint32_t o = sorted ? sorted[list[i]] : list[i];
   status[o] |= flag;
   if (stats) stats[o].status |= flag;
 }

@@ -68,6 +68,9 @@ struct CellArgs {
   double lo[3], hi[3];     // container box
   const uint32_t* work;    // grid positions to compute (nullptr: 0..nwork-1)
   int nwork;
+  const double* ext;       // nullable: sites not in the set, AoS; work items
+                           // then index these sites and the outputs
+                           // (Diagram::get_cell_at_particle, no self)
   int xcd_chunk;           // > 0: XCDs take runs of this many consecutive cells
   const int64_t* groups;   // nullable
   int64_t target_group;    // < 0: none

@@ -37,8 +37,11 @@ EXPORTS = [
     "tess_diagram_compute_scheduled", "tess_diagram_features", "tess_mlp_forward", "tess_mlp_gradient",
     "tess_mfma_probe", "tess_batch_create", "tess_batch_destroy", "tess_batch_sets", "tess_batch_set_config",
     "tess_batch_get_config", "tess_batch_compute", "tess_batch_tune", "tess_batch_total_faces",
-    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith", "tess_diagram_query", "tess_diagram_query_device", "tess_query_count",
-    "tess_query_total", "tess_query_copy", "tess_query_device", "tess_query_timings", "tess_query_destroy",
+    "tess_batch_copy_cells", "tess_batch_timings", "tess_diag_arith", "tess_diagram_query",
+    "tess_diagram_query_device", "tess_query_count", "tess_query_total", "tess_query_copy", "tess_query_device",
+    "tess_query_timings", "tess_query_destroy", "tess_diagram_cells_at_particles", "tess_site_cells_count",
+    "tess_site_cells_total_faces", "tess_site_cells_copy", "tess_site_cells_copy_stats", "tess_site_cells_timings",
+    "tess_site_cells_destroy",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 MLP_IN, MLP_PARAMS = 32, 7312
@@ -148,6 +151,13 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_diag_arith": (C.c_int, [vp, dp, dp, sz, dp, dp, dp]),
         "tess_diagram_query": (C.c_int, [vp, C.c_int, dp, sz, dp, sz, C.c_int64, C.POINTER(vp)]),
         "tess_diagram_query_device": (C.c_int, [vp, C.c_int, vp, sz, vp, sz, C.c_int64, C.POINTER(vp)]),
+        "tess_diagram_cells_at_particles": (C.c_int, [vp, dp, sz, C.c_double, C.c_int64, C.POINTER(vp)]),
+        "tess_site_cells_count": (sz, [vp]),
+        "tess_site_cells_total_faces": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+        "tess_site_cells_copy": (C.c_int, [vp, dp, dp, dp, dp, dp]),
+        "tess_site_cells_copy_stats": (C.c_int, [vp, dp]),
+        "tess_site_cells_timings": (C.c_int, [vp, dp, C.POINTER(C.c_uint32)]),
+        "tess_site_cells_destroy": (None, [vp]),
         "tess_query_count": (sz, [vp]),
         "tess_query_total": (C.c_uint64, [vp]),
         "tess_query_copy": (C.c_int, [vp, dp, dp]),
@@ -469,6 +479,33 @@ class Diagram:
         """Cell::compute_neighbor_cloud (interface.rs:348-365) of the cell of
         point `index` (its position is the point's; host-created diagrams)."""
         return self.query(QUERY_CLOUD, self.xyz[index], radius, target_group)[1]
+
+    def cells_at_particles(self, sites, search_radius=None, target_group=None, stats=False):
+        """Diagram::get_cell_at_particle + compute_voronoi_cell for sites that
+        are not points of the diagram: dict as cells() (+ stats, timings)."""
+        q = np.ascontiguousarray(np.asarray(sites, np.float64).reshape(-1, 3))
+        r = math.nan if search_radius is None else float(search_radius)
+        g = -1 if target_group is None else int(target_group)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.tess_diagram_cells_at_particles(self.h, _ptr(q), len(q), r, g, C.byref(h)))
+        try:
+            m = len(q)
+            tf = C.c_uint64()
+            self.ctx.check(self.lib.tess_site_cells_total_faces(h, C.byref(tf)))
+            vol, fc = np.empty(m), np.empty(m, np.uint32)
+            fo, nb, ar = np.empty(m + 1, np.uint64), np.empty(tf.value, np.int32), np.empty(tf.value)
+            self.ctx.check(self.lib.tess_site_cells_copy(h, _ptr(vol), _ptr(fc), _ptr(fo), _ptr(nb), _ptr(ar)))
+            out = dict(volume=vol, face_count=fc, face_offset=fo, neighbor=nb, area=ar)
+            if stats:
+                st = np.empty(m, STATS_DTYPE)
+                self.ctx.check(self.lib.tess_site_cells_copy_stats(h, _ptr(st)))
+                out["stats"] = st
+            ms, rr = np.zeros(2), C.c_uint32()
+            self.ctx.check(self.lib.tess_site_cells_timings(h, _ptr(ms), C.byref(rr)))
+            out["timings"] = dict(cells_ms=float(ms[0]), compact_ms=float(ms[1]), capacity_reruns=rr.value)
+        finally:
+            self.lib.tess_site_cells_destroy(h)
+        return out
 
     def get_cell_at_index(self, index, search_radius=None, target_group=None):
         """interface.rs:186-208 + Cell::compute_voronoi_cell."""
