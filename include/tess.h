@@ -178,6 +178,22 @@ size_t tess_cell_face_vertices(const tess_cell* c, size_t face, double* xyz, siz
 /* Cell::compute_vertices (interface.rs:368-370), site-relative. */
 size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
 
+/* ---- geometry of every cell (interface.rs:367-370, :403-406) -----------
+ * Cell::compute_vertices and VoronoiFace::compute_vertices
+ * (polyhedron.rs:885-919) for all cells at once, as CSR.  Runs compute
+ * first if needed.  Vertices are site-relative, per cell in the kernel's
+ * slot order (the reference's pool order is not reproducible);
+ * vertex_offset[n + 1] indexes them.  Each face (in the order of the
+ * neighbour lists, so face f of the whole diagram is neighbor[f]) has a ring
+ * ring[ring_offset[f] .. ring_offset[f + 1]) of vertex numbers within its
+ * cell (global vertex = vertex_offset[cell] + entry), starting at the face's
+ * lexicographically smallest vertex and counter-clockwise seen from outside.
+ * ring_offset has total_faces + 1 entries. */
+tess_status tess_diagram_compute_geometry(tess_diagram* d);
+tess_status tess_diagram_geometry_sizes(const tess_diagram* d, uint64_t* nvertices, uint64_t* nring);
+tess_status tess_diagram_copy_geometry(const tess_diagram* d, uint64_t* vertex_offset, double* vertices,
+                                       uint64_t* ring_offset, uint16_t* ring);
+
 /* ---- cells of outside sites (interface.rs:211-232) ---------------------
  * Diagram::get_cell_at_particle + Cell::compute_voronoi_cell for m sites
  * that are not points of the diagram (xyz: host AoS): each cell is the

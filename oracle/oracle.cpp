@@ -957,7 +957,7 @@ template <class Cell>
 void compute_cell(Grid& G, const double* x, const double* y, const double* z, size_t i,
                   const double lo[3], const double hi[3], int mode, Cell& C,
                   std::vector<FaceOut>& faces, double* volume, orc_cell_stats* st, bool check,
-                  const Filter& flt, const V3* ext = nullptr) {
+                  const Filter& flt, const V3* ext = nullptr, std::vector<Ring>* rings_out = nullptr) {
   const V3 site = ext ? *ext : V3{x[i], y[i], z[i]};
   C.init(lo, hi, site);
   uint64_t flops = 24 + 40;
@@ -1043,6 +1043,18 @@ void compute_cell(Grid& G, const double* x, const double* y, const double* z, si
   C.rings(rings);
   *volume = ring_outputs(rings, &faces, &flops);
   std::sort(faces.begin(), faces.end(), [](const FaceOut& a, const FaceOut& b) { return a.tag < b.tag; });
+  if (rings_out) {
+    // compute_face_vertices (polyhedron.rs:895-919) per face in output (tag)
+    // order, each ring from its lexicographically smallest vertex
+    *rings_out = rings;
+    std::sort(rings_out->begin(), rings_out->end(), [](const Ring& a, const Ring& b) { return a.tag < b.tag; });
+    for (Ring& r : *rings_out) {
+      size_t a = 0;
+      for (size_t i = 1; i < r.v.size(); ++i)
+        if (lex_less(r.v[i], r.v[a])) a = i;
+      std::rotate(r.v.begin(), r.v.begin() + (long)a, r.v.end());
+    }
+  }
   s.new_vertices = cs.new_vertices;
   s.flops = flops;
   s.r2 = r2;
@@ -1060,6 +1072,8 @@ struct orc_poly {
 
 struct orc_result {
   size_t ncells = 0;
+  std::vector<uint64_t> ring_off;  // geometry: per face, into ring_xyz (3 doubles per entry)
+  std::vector<double> ring_xyz;
   std::vector<double> volume;
   std::vector<uint32_t> fcount;
   std::vector<uint64_t> foff;
@@ -1367,6 +1381,54 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
       R->area[R->foff[c] + k] = per[c][k].area;
     }
   return R;
+}
+
+orc_result* orc_compute_geometry(const double* x, const double* y, const double* z, size_t n, const double lo[3],
+                                 const double hi[3], double density, const uint32_t* cells, size_t ncells,
+                                 int nthreads) {
+  orc_result* R = orc_compute(x, y, z, n, lo, hi, density, ORC_MODE_SECURITY, ORC_CLIP_FACES, cells, ncells,
+                              nthreads);
+  if (n == 0) return R;
+  Grid G;
+  grid_setup(G, x, y, z, n, density, false);
+  const size_t nc = R->ncells;
+  std::vector<std::vector<Ring>> per(nc);
+  Filter flt;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    FaceCell F;
+    std::vector<FaceOut> faces;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 256)
+#endif
+    for (size_t c = 0; c < nc; ++c) {
+      double vol;
+      orc_cell_stats st;
+      compute_cell(G, x, y, z, cells ? cells[c] : c, lo, hi, ORC_MODE_SECURITY, F, faces, &vol, &st, false, flt,
+                   nullptr, &per[c]);
+    }
+  }
+  R->ring_off.assign(1, 0);
+  for (size_t c = 0; c < nc; ++c)
+    for (const Ring& r : per[c]) {
+      for (const V3& v : r.v) {
+        R->ring_xyz.push_back(v.x);
+        R->ring_xyz.push_back(v.y);
+        R->ring_xyz.push_back(v.z);
+      }
+      R->ring_off.push_back(R->ring_xyz.size() / 3);
+    }
+  return R;
+}
+
+uint64_t orc_result_ring_entries(const orc_result* R) { return R->ring_xyz.size() / 3; }
+
+void orc_result_copy_rings(const orc_result* R, uint64_t* ring_offset, double* xyz) {
+  if (ring_offset) std::copy(R->ring_off.begin(), R->ring_off.end(), ring_offset);
+  if (xyz) std::copy(R->ring_xyz.begin(), R->ring_xyz.end(), xyz);
 }
 size_t orc_result_cells(const orc_result* R) { return R->ncells; }
 uint64_t orc_result_total_faces(const orc_result* R) { return R->ncells ? R->foff[R->ncells] : 0; }

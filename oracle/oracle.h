@@ -153,6 +153,16 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
                                   const double lo[3], const double hi[3], double density, int mode, int clip,
                                   const double* sites, size_t nsites, int nthreads, double max_radius,
                                   const int64_t* groups, int64_t target_group);
+/* orc_compute (security stop, per-face clip) plus the geometry of every cell:
+ * the vertex ring of each face (compute_face_vertices, polyhedron.rs:895-919),
+ * faces in output order, each ring from its lexicographically smallest
+ * vertex, counter-clockwise seen from outside, site-relative. */
+orc_result* orc_compute_geometry(const double* x, const double* y, const double* z, size_t n, const double lo[3],
+                                 const double hi[3], double density, const uint32_t* cells, size_t ncells,
+                                 int nthreads);
+uint64_t orc_result_ring_entries(const orc_result* R);
+/* ring_offset: total faces + 1 entries (into the entries); xyz: 3 per entry */
+void orc_result_copy_rings(const orc_result* R, uint64_t* ring_offset, double* xyz);
 orc_result* orc_compute_query(const double* x, const double* y, const double* z, size_t n,
                               const double lo[3], const double hi[3], double density, int mode, int clip,
                               const uint32_t* cells, size_t ncells, int nthreads, double max_radius,

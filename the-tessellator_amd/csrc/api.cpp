@@ -303,6 +303,7 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   tess_status st;
   Events ev;
   if (!ev.ok) return c->fail(TESS_E_DEVICE, "event create failed");
+  d->geo_ready = false;
   (void)hipEventRecord(ev.e[0], s);
   if ((st = bin(d)) != TESS_OK) return st;
   (void)hipEventRecord(ev.e[1], s);

@@ -697,7 +697,7 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
 
 // One cell: the site at grid position gpos of the set A.  Returns the
 // cell's status bits (wave-uniform); the caller queues capacity reruns.
-template <int VPL, int PPL, int FPL>
+template <int VPL, int PPL, int FPL, bool GEO = false>
 __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos, Lds<VPL, PPL, FPL>& S,
                                               const int lane) {
   using L = Lds<VPL, PPL, FPL>;
@@ -1043,6 +1043,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     int32_t tagv[FPL];
     int seqv[FPL];
     bool fal[FPL];
+    int hanc[FPL], ringk[FPL];  // geometry export: anchor half-edge, ring length
 #pragma unroll
     for (int r = 0; r < FPL; ++r) {
       const int f = lane + 64 * r;
@@ -1051,6 +1052,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       contrib[r] = 0;
       tagv[r] = 0;
       seqv[r] = 0;
+      hanc[r] = 0;
+      ringk[r] = 0;
       if (fal[r]) {
         const int st = S.f_start[f];
         int h = st, ha = st, k = 0;
@@ -1088,6 +1091,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         area[r] = 0.5 * sqrt((wx * wx + wy * wy) + wz * wz);
         tagv[r] = S.f_tag[f];
         seqv[r] = S.f_seq[f];
+        hanc[r] = ha;
+        ringk[r] = k;
         oflops += 3ull * (k - 1) + 12ull * (k - 2) + 13;
       }
     }
@@ -1156,6 +1161,66 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         A.exp_meta[1] = nfaces;
       }
     }
+    if constexpr (GEO) {
+      // Geometry of whole sets (compute_vertices / compute_face_vertices,
+      // polyhedron.rs:885-919): the live vertices in slot order, and per face
+      // in output order the ring of local vertex numbers from the anchor (the
+      // lexicographically smallest vertex) along the face's orientation.
+      int nring = 0;
+#pragma unroll
+      for (int r = 0; r < FPL; ++r) nring += fal[r] ? ringk[r] : 0;
+      if (A.geo_count) {
+        nring = wave_incl_scan(nring, lane);
+        if (lane == 63) {
+          A.geo_count[oi] = (uint32_t)popc_multi(C.VA);
+          A.geo_count[(size_t)A.n + oi] = (uint32_t)nring;
+        }
+      } else {
+        const uint64_t vb = A.geo_voff[oi];
+        int below = 0;  // live slots in the lower registers
+#pragma unroll
+        for (int r = 0; r < VPL; ++r) {
+          if ((C.VA[r] >> lane) & 1ull) {
+            const uint64_t q = vb + (uint64_t)(below + mbcnt(C.VA[r], lane));
+            const int v = lane + 64 * r;
+            A.geo_v[3 * q] = S.vx[v];
+            A.geo_v[3 * q + 1] = S.vy[v];
+            A.geo_v[3 * q + 2] = S.vz[v];
+          }
+          below += popc(C.VA[r]);
+        }
+        // ring offsets in output order: lengths by rank, then a wave scan
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < FPL; ++r)
+          if (fal[r]) S.bpref[rt[r]] = (uint32_t)ringk[r];
+        wave_sync();
+        const int kk = lane < nfaces ? (int)S.bpref[lane] : 0;
+        const int ex = wave_incl_scan(kk, lane) - kk;
+        const uint64_t rb = A.geo_rbase[oi];
+        if (lane < nfaces) A.geo_roff[A.geo_foff[oi] + lane] = rb + (uint64_t)ex;
+        S.bstart[lane] = (uint32_t)ex;
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < FPL; ++r)
+          if (fal[r]) {
+            uint64_t pos = rb + S.bstart[rt[r]];
+            int h = hanc[r], j = 0;
+            do {
+              const int v = S.he_tgt[h];
+              const int vr = v >> 6;
+              const uint64_t lower = (1ull << (v & 63)) - 1ull;
+              int rank = 0;  // live slots below v
+#pragma unroll
+              for (int q = 0; q < VPL; ++q)
+                rank += q < vr ? popc(C.VA[q]) : (q == vr ? popc(C.VA[q] & lower) : 0);
+              A.geo_ring[pos + j] = (uint16_t)rank;
+              h = S.he_next[h];
+              ++j;
+            } while (h != hanc[r] && j < ringk[r]);
+          }
+      }
+    }
   } else {
     nfaces = 0;
   }
@@ -1193,14 +1258,14 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 }
 
 // One cell per wavefront over a set's grid positions (all, or a work list).
-template <int VPL, int PPL, int FPL>
+template <int VPL, int PPL, int FPL, bool GEO = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells(CellArgs A) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
   const int w = A.xcd_chunk ? (int)xcd_map(blockIdx.x, (uint32_t)A.xcd_chunk) : (int)blockIdx.x;
   if (w >= A.nwork) return;
   const int gpos = A.work ? (int)A.work[w] : w;
-  const uint32_t status = cell_body(A, gpos, S, lane);
+  const uint32_t status = cell_body<VPL, PPL, FPL, GEO>(A, gpos, S, lane);
   if (lane == 0 && (status & TESS_CELL_CAPACITY) && A.redo) {
     const uint32_t slot = atomicAdd(A.nredo, 1u);
     A.redo[slot] = (uint32_t)gpos;
@@ -1323,6 +1388,13 @@ hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
     default:
       return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_cells_geometry(const CellArgs& a, hipStream_t s) {
+  if (a.nwork <= 0) return hipSuccess;
+  const int nb = a.xcd_chunk ? (int)xcd_grid((uint32_t)a.nwork, (uint32_t)a.xcd_chunk) : a.nwork;
+  hipLaunchKernelGGL((k_cells<4, 6, 2, true>), dim3(nb), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

@@ -158,6 +158,13 @@ struct tess_diagram {
   DevBuf<uint32_t> sched_key, sched_list, sched_hist, sched_sums;
   DevBuf<double> exp_v;
   DevBuf<int32_t> exp_ring, exp_meta;
+  // geometry of every cell (tess_diagram_compute_geometry)
+  DevBuf<uint32_t> geo_count;
+  DevBuf<uint64_t> geo_voff, geo_rbase, geo_roff, geo_sums;
+  DevBuf<double> geo_v;
+  DevBuf<uint16_t> geo_ring;
+  uint64_t geo_nv = 0, geo_nr = 0;
+  bool geo_ready = false;
   uint64_t total_faces = 0;
   tess_timings tim;
   uint64_t degenerate = 0;
@@ -205,5 +212,7 @@ using Planner = std::function<tess_status(tess_diagram*, Schedule&)>;
 // Binning, shell table, cell passes (with the capacity rerun), face offsets
 // and compaction; fills the diagram's timings.  plan == nullptr: plain.
 tess_status compute_cells(tess_diagram* d, const Planner* plan);
+// XCD run length of whole-set cell passes (the caller's, else 256)
+int plain_xcd_chunk(const tess_diagram* d);
 
 }  // namespace tess

@@ -41,7 +41,8 @@ EXPORTS = [
     "tess_diagram_query_device", "tess_query_count", "tess_query_total", "tess_query_copy", "tess_query_device",
     "tess_query_timings", "tess_query_destroy", "tess_diagram_cells_at_particles", "tess_site_cells_count",
     "tess_site_cells_total_faces", "tess_site_cells_copy", "tess_site_cells_copy_stats", "tess_site_cells_timings",
-    "tess_site_cells_destroy",
+    "tess_site_cells_destroy", "tess_diagram_compute_geometry", "tess_diagram_geometry_sizes",
+    "tess_diagram_copy_geometry",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 MLP_IN, MLP_PARAMS = 32, 7312
@@ -152,6 +153,9 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_diagram_query": (C.c_int, [vp, C.c_int, dp, sz, dp, sz, C.c_int64, C.POINTER(vp)]),
         "tess_diagram_query_device": (C.c_int, [vp, C.c_int, vp, sz, vp, sz, C.c_int64, C.POINTER(vp)]),
         "tess_diagram_cells_at_particles": (C.c_int, [vp, dp, sz, C.c_double, C.c_int64, C.POINTER(vp)]),
+        "tess_diagram_compute_geometry": (C.c_int, [vp]),
+        "tess_diagram_geometry_sizes": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "tess_diagram_copy_geometry": (C.c_int, [vp, dp, dp, dp, dp]),
         "tess_site_cells_count": (sz, [vp]),
         "tess_site_cells_total_faces": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
         "tess_site_cells_copy": (C.c_int, [vp, dp, dp, dp, dp, dp]),
@@ -479,6 +483,19 @@ class Diagram:
         """Cell::compute_neighbor_cloud (interface.rs:348-365) of the cell of
         point `index` (its position is the point's; host-created diagrams)."""
         return self.query(QUERY_CLOUD, self.xyz[index], radius, target_group)[1]
+
+    def geometry(self):
+        """Vertices and face rings of every cell (tess_diagram_compute_geometry):
+        vertex_offset [n + 1], vertices [V, 3] (site-relative), ring_offset
+        [faces + 1], ring (vertex numbers within the cell)."""
+        self.ctx.check(self.lib.tess_diagram_compute_geometry(self.h))
+        nv, nr, tf = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.lib.tess_diagram_geometry_sizes(self.h, C.byref(nv), C.byref(nr)))
+        self.ctx.check(self.lib.tess_diagram_total_faces(self.h, C.byref(tf)))
+        vo, v = np.empty(self.n + 1, np.uint64), np.empty((nv.value, 3))
+        ro, ring = np.empty(tf.value + 1, np.uint64), np.empty(nr.value, np.uint16)
+        self.ctx.check(self.lib.tess_diagram_copy_geometry(self.h, _ptr(vo), _ptr(v), _ptr(ro), _ptr(ring)))
+        return dict(vertex_offset=vo, vertices=v, ring_offset=ro, ring=ring)
 
     def cells_at_particles(self, sites, search_radius=None, target_group=None, stats=False):
         """Diagram::get_cell_at_particle + compute_voronoi_cell for sites that
