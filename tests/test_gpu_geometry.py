@@ -25,9 +25,16 @@ def _ring_coords(d, cells, geo):
 @pytest.mark.parametrize("workload", ["uniform", "clustered"])
 def test_rings_match_oracle(ctx, workload):
     pts = tess.uniform(20000, 81) if workload == "uniform" else tess.clustered(20000, 82)
-    d = tess.Diagram(ctx, pts, LO, HI, density=5.0)
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0, collect_stats=True)
+    d.compute()
+    before = (d.cells(), d.stats())
     geo = d.geometry()
     cells = d.cells()
+    for k in before[0]:  # the geometry passes leave the cell outputs and counters alone
+        assert np.array_equal(before[0][k], cells[k]), k
+    assert np.array_equal(before[1], d.stats())
+    if workload == "clustered":
+        assert d.timings()["capacity_reruns"] > 0  # the capacity-256 geometry rerun is exercised
     ref = orc.compute_geometry(pts, LO, HI, density=5.0)
     assert np.array_equal(cells["neighbor"], ref["neighbor"])
     assert np.array_equal(geo["ring_offset"], ref["ring_offset"])
@@ -81,7 +88,8 @@ def test_geometry_follows_recompute(ctx):
     with pytest.raises(tess.TessError):
         _geometry_sizes(d)
     g2 = d.geometry()
-    for k in g1:
+    assert g2["timings"]["count_ms"] > 0 and g2["timings"]["fill_ms"] > 0
+    for k in ("vertex_offset", "vertices", "ring_offset", "ring"):
         assert np.array_equal(g1[k], g2[k]), k
 
 

@@ -1122,18 +1122,21 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     for (int r = 0; r < FPL; ++r)
       if (fal[r]) S.u.fcontrib[rs[r]] = contrib[r];
     wave_sync();
-    if (lane == 0) {
+    // (the geometry passes write only the geometry)
+    if (!GEO && lane == 0) {
       double vol = 0.0;
       for (int f = 0; f < nfaces; ++f) vol = vol + S.u.fcontrib[f];
       A.volume[oi] = vol / 6.0;
       A.fcount[oi] = (uint32_t)nfaces;
     }
+    if (!GEO) {
 #pragma unroll
-    for (int r = 0; r < FPL; ++r)
-      if (fal[r]) {
-        A.snbr[(size_t)oi * kFaceStage + rt[r]] = tagv[r];
-        A.sarea[(size_t)oi * kFaceStage + rt[r]] = area[r];
-      }
+      for (int r = 0; r < FPL; ++r)
+        if (fal[r]) {
+          A.snbr[(size_t)oi * kFaceStage + rt[r]] = tagv[r];
+          A.sarea[(size_t)oi * kFaceStage + rt[r]] = area[r];
+        }
+    }
     if (A.exp_v) {
       // geometry export for a single-cell query: slots and rings in tag order
 #pragma unroll
@@ -1226,11 +1229,13 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   }
   oflops = wave_sum_u64(oflops);
   if (lane == 0) {
-    if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) {
-      A.volume[oi] = 0.0;
-      A.fcount[oi] = 0;
+    if (!GEO) {
+      if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) {
+        A.volume[oi] = 0.0;
+        A.fcount[oi] = 0;
+      }
+      A.status[oi] = status;
     }
-    A.status[oi] = status;
     if (A.summary) {
       if (!(status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE))) {
         atomicAdd(&A.summary->flops, (unsigned long long)(flops + oflops + 1));
@@ -1240,7 +1245,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       if (status & TESS_CELL_CAPACITY) atomicAdd(&A.summary->capacity, 1u);
       if (status & TESS_CELL_TABLE) atomicAdd(&A.summary->table, 1u);
     }
-    if (A.stats) {
+    if (!GEO && A.stats) {
       tess_cell_stats st;
       st.streamed = streamed;
       st.classified = classified;
@@ -1391,10 +1396,15 @@ hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_cells_geometry(const CellArgs& a, hipStream_t s) {
+hipError_t launch_cells_geometry(int vpl, const CellArgs& a, hipStream_t s) {
   if (a.nwork <= 0) return hipSuccess;
   const int nb = a.xcd_chunk ? (int)xcd_grid((uint32_t)a.nwork, (uint32_t)a.xcd_chunk) : a.nwork;
-  hipLaunchKernelGGL((k_cells<4, 6, 2, true>), dim3(nb), dim3(64), 0, s, a);
+  if (vpl == 1)
+    hipLaunchKernelGGL((k_cells<1, 2, 1, true>), dim3(nb), dim3(64), 0, s, a);
+  else if (vpl == 4)
+    hipLaunchKernelGGL((k_cells<4, 6, 2, true>), dim3(nb), dim3(64), 0, s, a);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -24,17 +24,50 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   if (d->geo_count.alloc(2 * n) != hipSuccess || d->geo_voff.alloc(n + 1) != hipSuccess ||
       d->geo_rbase.alloc(n + 1) != hipSuccess || d->geo_sums.alloc((n + 1023) / 1024 + 1) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  // count pass: live vertices and ring entries per cell (cells recomputed at
-  // capacity 256, so no rerun; their other outputs are rewritten unchanged)
+  // count pass: live vertices and ring entries per cell.  The cells are
+  // recomputed like compute does it: capacity 64 first, the overflows again
+  // at 256 (the geometry kernels write nothing else).
   CellArgs a = base_args(d, t);
   a.work = nullptr;
   a.nwork = (int)n;
   a.xcd_chunk = plain_xcd_chunk(d);
   a.stats = nullptr;
   a.summary = nullptr;
+  a.redo = d->redo.p + 1;
+  a.nredo = d->redo.p;
   a.geo_count = d->geo_count.p;
+  CellArgs r = a;  // the rerun of the overflows
+  r.work = d->redo_list.p;
+  r.xcd_chunk = 0;
+  r.redo = nullptr;
+  r.nredo = nullptr;
+  uint32_t nredo = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return c->fail(TESS_E_DEVICE, "event create failed");
+  struct Release {
+    hipEvent_t* e;
+    ~Release() {
+      for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e[k]);
+    }
+  } release{ev};
   if ((st = c->hip(launch_fill_u32(d->geo_count.p, 2 * n, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_cells_geometry(a, s), "geometry count")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev[0], s), "event")) != TESS_OK ||
+      (st = c->hip(launch_cells_geometry(1, a, s), "geometry count")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "geometry count")) != TESS_OK)
+    return st;
+  if (nredo) {
+    if (d->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    r.work = d->redo_list.p;
+    r.nwork = (int)nredo;
+    if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
+                     "redo")) != TESS_OK ||
+        (st = c->hip(launch_cells_geometry(4, r, s), "geometry count")) != TESS_OK)
+      return st;
+  }
+  if ((st = c->hip(hipEventRecord(ev[1], s), "event")) != TESS_OK ||
       (st = c->hip(launch_face_offsets(d->geo_count.p, (int)n, d->geo_voff.p, d->geo_sums.p, s), "scan")) !=
           TESS_OK ||
       (st = c->hip(launch_face_offsets(d->geo_count.p + n, (int)n, d->geo_rbase.p, d->geo_sums.p, s), "scan")) !=
@@ -49,19 +82,32 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   if (d->geo_v.alloc(3 * d->geo_nv) != hipSuccess || d->geo_roff.alloc(F + 1) != hipSuccess ||
       d->geo_ring.alloc(d->geo_nr) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  // fill pass at the offsets
-  a.geo_count = nullptr;
-  a.geo_voff = d->geo_voff.p;
-  a.geo_foff = d->foff.p;
-  a.geo_rbase = d->geo_rbase.p;
-  a.geo_v = d->geo_v.p;
-  a.geo_roff = d->geo_roff.p;
-  a.geo_ring = d->geo_ring.p;
-  if ((st = c->hip(launch_cells_geometry(a, s), "geometry fill")) != TESS_OK ||
+  // fill pass at the offsets (the overflows write nothing at 64 and are
+  // filled by the rerun)
+  for (CellArgs* p : {&a, &r}) {
+    p->geo_count = nullptr;
+    p->geo_voff = d->geo_voff.p;
+    p->geo_foff = d->foff.p;
+    p->geo_rbase = d->geo_rbase.p;
+    p->geo_v = d->geo_v.p;
+    p->geo_roff = d->geo_roff.p;
+    p->geo_ring = d->geo_ring.p;
+  }
+  a.redo = nullptr;
+  a.nredo = nullptr;
+  float ms[2] = {0, 0};
+  if ((st = c->hip(hipEventRecord(ev[2], s), "event")) != TESS_OK ||
+      (st = c->hip(launch_cells_geometry(1, a, s), "geometry fill")) != TESS_OK ||
+      (nredo && (st = c->hip(launch_cells_geometry(4, r, s), "geometry fill")) != TESS_OK) ||
+      (st = c->hip(hipEventRecord(ev[3], s), "event")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(d->geo_roff.p + F, &d->geo_nr, 8, hipMemcpyHostToDevice, s), "upload")) !=
           TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "geometry fill")) != TESS_OK)
+      (st = c->hip(hipStreamSynchronize(s), "geometry fill")) != TESS_OK ||
+      (st = c->hip(hipEventElapsedTime(&ms[0], ev[0], ev[1]), "event")) != TESS_OK ||
+      (st = c->hip(hipEventElapsedTime(&ms[1], ev[2], ev[3]), "event")) != TESS_OK)
     return st;
+  d->geo_ms[0] = ms[0];
+  d->geo_ms[1] = ms[1];
   d->geo_ready = true;
   return TESS_OK;
 }
@@ -71,6 +117,14 @@ tess_status tess_diagram_geometry_sizes(const tess_diagram* d, uint64_t* nvertic
   if (!d->geo_ready) return d->ctx->fail(TESS_E_STATE, "geometry read before tess_diagram_compute_geometry");
   if (nvertices) *nvertices = d->geo_nv;
   if (nring) *nring = d->geo_nr;
+  return TESS_OK;
+}
+
+tess_status tess_diagram_geometry_timings(const tess_diagram* d, double ms[2]) {
+  if (!d || !ms) return TESS_E_INVALID;
+  if (!d->geo_ready) return d->ctx->fail(TESS_E_STATE, "geometry read before tess_diagram_compute_geometry");
+  ms[0] = d->geo_ms[0];
+  ms[1] = d->geo_ms[1];
   return TESS_OK;
 }
 

@@ -40,8 +40,9 @@ hipError_t launch_invert(const uint32_t* sorted, int n, uint32_t* inv, hipStream
 
 // cell.hip: vpl = vertex slots per lane (1, 2 or 4)
 hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s);
-// the same cells with the geometry export (capacity 256; CellArgs::geo_*)
-hipError_t launch_cells_geometry(const CellArgs& a, hipStream_t s);
+// the same cells with the geometry export only (CellArgs::geo_*; the other
+// outputs are not written), vpl 1 or 4
+hipError_t launch_cells_geometry(int vpl, const CellArgs& a, hipStream_t s);
 // many sets: one wavefront per item of a list of host-known length n ...
 hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream_t s);
 // ... or persistent over a device-counted list at the largest capacity

@@ -164,6 +164,7 @@ struct tess_diagram {
   DevBuf<double> geo_v;
   DevBuf<uint16_t> geo_ring;
   uint64_t geo_nv = 0, geo_nr = 0;
+  double geo_ms[2] = {0, 0};  // count pass, fill pass
   bool geo_ready = false;
   uint64_t total_faces = 0;
   tess_timings tim;

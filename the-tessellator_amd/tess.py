@@ -42,7 +42,7 @@ EXPORTS = [
     "tess_query_timings", "tess_query_destroy", "tess_diagram_cells_at_particles", "tess_site_cells_count",
     "tess_site_cells_total_faces", "tess_site_cells_copy", "tess_site_cells_copy_stats", "tess_site_cells_timings",
     "tess_site_cells_destroy", "tess_diagram_compute_geometry", "tess_diagram_geometry_sizes",
-    "tess_diagram_copy_geometry",
+    "tess_diagram_copy_geometry", "tess_diagram_geometry_timings",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 MLP_IN, MLP_PARAMS = 32, 7312
@@ -156,6 +156,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_diagram_compute_geometry": (C.c_int, [vp]),
         "tess_diagram_geometry_sizes": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "tess_diagram_copy_geometry": (C.c_int, [vp, dp, dp, dp, dp]),
+        "tess_diagram_geometry_timings": (C.c_int, [vp, dp]),
         "tess_site_cells_count": (sz, [vp]),
         "tess_site_cells_total_faces": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
         "tess_site_cells_copy": (C.c_int, [vp, dp, dp, dp, dp, dp]),
@@ -495,7 +496,10 @@ class Diagram:
         vo, v = np.empty(self.n + 1, np.uint64), np.empty((nv.value, 3))
         ro, ring = np.empty(tf.value + 1, np.uint64), np.empty(nr.value, np.uint16)
         self.ctx.check(self.lib.tess_diagram_copy_geometry(self.h, _ptr(vo), _ptr(v), _ptr(ro), _ptr(ring)))
-        return dict(vertex_offset=vo, vertices=v, ring_offset=ro, ring=ring)
+        ms = np.zeros(2)
+        self.ctx.check(self.lib.tess_diagram_geometry_timings(self.h, _ptr(ms)))
+        return dict(vertex_offset=vo, vertices=v, ring_offset=ro, ring=ring,
+                    timings=dict(count_ms=float(ms[0]), fill_ms=float(ms[1])))
 
     def cells_at_particles(self, sites, search_radius=None, target_group=None, stats=False):
         """Diagram::get_cell_at_particle + compute_voronoi_cell for sites that
