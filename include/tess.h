@@ -62,7 +62,21 @@ typedef struct {
   int32_t collect_stats;   /* 1: keep per-cell search counters */
   int32_t xcd_chunk;       /* whole-set passes: cells each XCD takes in a row
                               (0: automatic, 256) */
+  int32_t point_order;     /* TESS_ORDER_*: the order of a grid cell's points
+                              in the candidate stream */
+  int32_t reserved;
 } tess_opts;
+
+enum {
+  /* ascending input index (the default) */
+  TESS_ORDER_INPUT = 0,
+  /* the reference's internal order: Diagram::sort_particles by
+   * morton_index (interface.rs:88-183; 10 levels on a 1024 grid over the
+   * points' bounding box grown from the origin), ties by input index.  The
+   * Celery is built over the sorted points (interface.rs:64), so a cell's
+   * candidates come in this order.  Ids stay original indices. */
+  TESS_ORDER_MORTON = 1
+};
 
 /* Per-cell search counters (collect_stats = 1). */
 typedef struct {
@@ -156,6 +170,12 @@ tess_status tess_diagram_grid(const tess_diagram* d, double bounds[6], double si
  * delimiters[cpd^3 + 1] (pass NULL to skip). */
 tess_status tess_diagram_copy_grid(const tess_diagram* d, uint32_t* cell_of_point,
                                    uint32_t* sorted_indices, uint64_t* delimiters);
+/* Diagram::original_indices / internal_indices (interface.rs:35-39, set by
+ * sort_particles, :88-148): the internal order is the Morton order of
+ * TESS_ORDER_MORTON (whatever point_order the diagram uses); either output
+ * may be NULL.  Computed on the device (Morton codes + radix sort). */
+tess_status tess_diagram_copy_index_maps(tess_diagram* d, uint32_t* original_of_internal,
+                                         uint32_t* internal_of_original);
 
 /* ---- one cell (interface.rs:186-208, :257-390) -------------------------
  * search_radius: NaN for None (expanding search with the security-radius

@@ -722,6 +722,67 @@ void orc_grid_build(const double* x, const double* y, const double* z, size_t n,
   for (size_t i = 0; i < n; ++i) sorted[cnt[bins[i]]++] = (uint32_t)i;
 }
 
+// Diagram::morton_index (interface.rs:151-183): 10 levels on a 1024 grid over
+// the diagram's bounding box, which starts as the default box at the origin
+// and is grown to contain every point (interface.rs:52-57,
+// vector3.rs:253-287).  The index per axis is `(offset / side) as usize`
+// (saturating, NaN -> 0); bit 3l holds x's bit l, 3l+1 y's, 3l+2 z's.
+void orc_morton_codes(const double* x, const double* y, const double* z, size_t n, uint32_t* codes) {
+  double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+  for (size_t i = 0; i < n; ++i) {
+    const double v[3] = {x[i], y[i], z[i]};
+    for (int a = 0; a < 3; ++a) {
+      if (v[a] < lo[a]) lo[a] = v[a];
+      if (v[a] > hi[a]) hi[a] = v[a];
+    }
+  }
+  double side[3];
+  for (int a = 0; a < 3; ++a) side[a] = (hi[a] - lo[a]) * (1.0 / 1024.0);
+  for (size_t i = 0; i < n; ++i) {
+    const double v[3] = {x[i], y[i], z[i]};
+    uint64_t idx[3];
+    for (int a = 0; a < 3; ++a) idx[a] = (uint64_t)trunc_sat((v[a] - lo[a]) / side[a]);
+    uint32_t m = 0;
+    int bit = 0;
+    for (int level = 0; level < 10; ++level)
+      for (int a = 0; a < 3; ++a) m |= (uint32_t)((idx[a] >> level) & 1u) << bit++;
+    codes[i] = m;
+  }
+}
+
+// sort_particles (interface.rs:88-148) as intended: internal order = points by
+// Morton code, ties by input index (the reference's sort_unstable_by leaves
+// them unspecified, and its in-place permutation loop indexes with codes).
+void orc_index_maps(const double* x, const double* y, const double* z, size_t n, uint32_t* original_of_internal,
+                    uint32_t* internal_of_original) {
+  std::vector<uint32_t> code(n), ord(n);
+  orc_morton_codes(x, y, z, n, code.data());
+  for (size_t i = 0; i < n; ++i) ord[i] = (uint32_t)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+  for (size_t k = 0; k < n; ++k) {
+    if (original_of_internal) original_of_internal[k] = ord[k];
+    if (internal_of_original) internal_of_original[ord[k]] = (uint32_t)k;
+  }
+}
+
+// orc_grid_build with the points of a cell in internal (Morton) order when
+// order == 1: the Celery of the reference is built over the sorted points
+// (interface.rs:64, celery.rs:357-369), so its sorted indices ascend by
+// internal index.  Ids stay original indices.
+void orc_grid_build_ordered(const double* x, const double* y, const double* z, size_t n, const orc_grid* g,
+                            int order, uint32_t* bins, uint32_t* sorted, uint64_t* delim) {
+  orc_grid_build(x, y, z, n, g, bins, sorted, delim);
+  if (order != 1) return;
+  std::vector<uint32_t> ooi(n);
+  orc_index_maps(x, y, z, n, ooi.data(), nullptr);
+  const uint64_t nb = (uint64_t)g->cpd * g->cpd * g->cpd;
+  std::vector<uint64_t> cur(delim, delim + nb);
+  for (size_t k = 0; k < n; ++k) {
+    const uint32_t i = ooi[k];
+    sorted[cur[bins[i]]++] = i;
+  }
+}
+
 // celery.rs:708-743.  sq(i sx) + sq(j sy) + sq(k sz) of the per-axis cell
 // gaps max(0, |a - b| - 1), against radius * radius.
 int orc_check_cell_in_range(const orc_grid* g, double x, double y, double z, double radius, int64_t i, int64_t j,
@@ -863,13 +924,13 @@ namespace {
 constexpr int64_t kTableL = 16;
 
 void grid_setup(Grid& G, const double* x, const double* y, const double* z, size_t n, double density,
-                bool want_full) {
+                bool want_full, int order = 0) {
   orc_grid_info(x, y, z, n, density, &G.g);
   const uint64_t nb = (uint64_t)G.g.cpd * G.g.cpd * G.g.cpd;
   std::vector<uint32_t> bins(n);
   G.sorted.resize(n);
   G.delim.resize(nb + 1);
-  orc_grid_build(x, y, z, n, &G.g, bins.data(), G.sorted.data(), G.delim.data());
+  orc_grid_build_ordered(x, y, z, n, &G.g, order, bins.data(), G.sorted.data(), G.delim.data());
   const int64_t L = want_full ? G.g.cpd - 1 : std::min<int64_t>(G.g.cpd - 1, kTableL);
   size_t m = orc_search_order(G.g.cpd, G.g.sx, G.g.sy, G.g.sz, L, nullptr, nullptr, nullptr, nullptr);
   G.key.resize(m);
@@ -1275,6 +1336,14 @@ orc_result* orc_compute_query(const double* x, const double* y, const double* z,
                               const double lo[3], const double hi[3], double density, int mode, int clip,
                               const uint32_t* cells, size_t ncells, int nthreads, double max_radius,
                               const int64_t* groups, int64_t target_group) {
+  return orc_compute_ordered(x, y, z, n, lo, hi, density, mode, clip, cells, ncells, nthreads, max_radius, groups,
+                             target_group, 0);
+}
+
+orc_result* orc_compute_ordered(const double* x, const double* y, const double* z, size_t n,
+                                const double lo[3], const double hi[3], double density, int mode, int clip,
+                                const uint32_t* cells, size_t ncells, int nthreads, double max_radius,
+                                const int64_t* groups, int64_t target_group, int order) {
   Filter flt;
   flt.max_radius = max_radius;
   flt.groups = groups;
@@ -1283,7 +1352,7 @@ orc_result* orc_compute_query(const double* x, const double* y, const double* z,
   if (n == 0) return R;
   auto t0 = std::chrono::steady_clock::now();
   Grid G;
-  grid_setup(G, x, y, z, n, density, mode == ORC_MODE_ALL_POINTS);
+  grid_setup(G, x, y, z, n, density, mode == ORC_MODE_ALL_POINTS, order);
   if (!cells) ncells = n;
   R->ncells = ncells;
   R->volume.resize(ncells);

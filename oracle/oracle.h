@@ -67,6 +67,17 @@ uint64_t orc_generation_index(int64_t cpd, int32_t ox, int32_t oy, int32_t oz);
 
 /* Candidate stream of a site: expand_all_no_radius (max_radius NaN, all n
  * points) or expand_all_in_radius(max_radius). Returns the count. */
+/* Diagram::morton_index (interface.rs:151-183) of every point: 10 levels on a
+ * 1024 grid over the points' bounding box grown from the origin. */
+void orc_morton_codes(const double* x, const double* y, const double* z, size_t n, uint32_t* codes);
+/* sort_particles (interface.rs:88-148) as intended: internal order by Morton
+ * code, ties by input index; either output may be NULL. */
+void orc_index_maps(const double* x, const double* y, const double* z, size_t n, uint32_t* original_of_internal,
+                    uint32_t* internal_of_original);
+/* orc_grid_build; order 1: a cell's points in internal (Morton) order. */
+void orc_grid_build_ordered(const double* x, const double* y, const double* z, size_t n, const orc_grid* g,
+                            int order, uint32_t* bins, uint32_t* sorted, uint64_t* delim);
+
 /* Radius queries (celery.rs:700-855) on a built grid.  check_cell_in_range
  * (celery.rs:708-743): is grid cell (i, j, k) within radius of the cell of
  * (x, y, z), counting touching cells as distance 0.  find_cells_in_radius
@@ -163,6 +174,12 @@ orc_result* orc_compute_geometry(const double* x, const double* y, const double*
 uint64_t orc_result_ring_entries(const orc_result* R);
 /* ring_offset: total faces + 1 entries (into the entries); xyz: 3 per entry */
 void orc_result_copy_rings(const orc_result* R, uint64_t* ring_offset, double* xyz);
+/* orc_compute_query with the candidate order of a grid cell chosen by order
+ * (0: input index, 1: internal Morton order, see orc_grid_build_ordered). */
+orc_result* orc_compute_ordered(const double* x, const double* y, const double* z, size_t n,
+                                const double lo[3], const double hi[3], double density, int mode, int clip,
+                                const uint32_t* cells, size_t ncells, int nthreads, double max_radius,
+                                const int64_t* groups, int64_t target_group, int order);
 orc_result* orc_compute_query(const double* x, const double* y, const double* z, size_t n,
                               const double lo[3], const double hi[3], double density, int mode, int clip,
                               const uint32_t* cells, size_t ncells, int nthreads, double max_radius,

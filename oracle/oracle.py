@@ -70,6 +70,11 @@ def lib():
             "orc_find_cells_in_radius": (sz, [C.POINTER(Grid)] + [C.c_double] * 4 + [dp]),
             "orc_find_neighbors": (sz, [dp, dp, dp, C.POINTER(Grid), dp, dp] + [C.c_double] * 4 + [C.c_int, dp]),
             "orc_queries": (None, [C.c_int, dp, dp, dp, sz, C.c_double, dp, C.c_int64, dp, sz, dp, dp, dp]),
+            "orc_morton_codes": (None, [dp, dp, dp, sz, dp]),
+            "orc_index_maps": (None, [dp, dp, dp, sz, dp, dp]),
+            "orc_grid_build_ordered": (None, [dp, dp, dp, sz, C.POINTER(Grid), C.c_int, dp, dp, dp]),
+            "orc_compute_ordered": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int,
+                                         C.c_double, dp, C.c_int64, C.c_int]),
             "orc_poly_new": (vp, [C.c_double] * 6),
             "orc_poly_new_faces": (vp, [C.c_double] * 6),
             "orc_poly_free": (None, [vp]),
@@ -158,7 +163,10 @@ def soa(xyz):
     return [np.ascontiguousarray(xyz[:, k]) for k in range(3)]
 
 
-def grid(xyz, density=1.25):
+ORDER_INPUT, ORDER_MORTON = 0, 1
+
+
+def grid(xyz, density=1.25, order=ORDER_INPUT):
     """(grid info, bins, sorted_indices, delimiters)."""
     x, y, z = soa(xyz)
     n = len(x)
@@ -166,8 +174,24 @@ def grid(xyz, density=1.25):
     lib().orc_grid_info(_p(x), _p(y), _p(z), n, density, C.byref(g))
     nb = g.cpd ** 3
     bins, srt, dl = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(nb + 1, np.uint64)
-    lib().orc_grid_build(_p(x), _p(y), _p(z), n, C.byref(g), _p(bins), _p(srt), _p(dl))
+    lib().orc_grid_build_ordered(_p(x), _p(y), _p(z), n, C.byref(g), order, _p(bins), _p(srt), _p(dl))
     return g, bins, srt, dl
+
+
+def morton_codes(xyz):
+    """Diagram::morton_index of every point (interface.rs:151-183)."""
+    x, y, z = soa(xyz)
+    out = np.empty(len(x), np.uint32)
+    lib().orc_morton_codes(_p(x), _p(y), _p(z), len(x), _p(out))
+    return out
+
+
+def index_maps(xyz):
+    """(original_of_internal, internal_of_original) of sort_particles."""
+    x, y, z = soa(xyz)
+    a, b = np.empty(len(x), np.uint32), np.empty(len(x), np.uint32)
+    lib().orc_index_maps(_p(x), _p(y), _p(z), len(x), _p(a), _p(b))
+    return a, b
 
 
 def axis_index(v, vmin, vmax, inv, cpd):
@@ -285,7 +309,7 @@ class Poly:
 
 # ---- cells (interface.rs) ------------------------------------------------
 def compute(xyz, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, cells=None, threads=0,
-            max_radius=None, groups=None, target_group=None):
+            max_radius=None, groups=None, target_group=None, order=0):
     """All (or the listed) cells: dict(volume, face_count, face_offset, neighbor, area, stats, seconds)."""
     x, y, z = soa(xyz)
     n = len(x)
@@ -293,9 +317,9 @@ def compute(xyz, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, cell
     cl = None if cells is None else np.ascontiguousarray(cells, np.uint32)
     nc = n if cl is None else len(cl)
     g = None if groups is None else np.ascontiguousarray(groups, np.int64)
-    r = lib().orc_compute_query(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(cl), nc, threads,
-                                float("nan") if max_radius is None else float(max_radius), _p(g),
-                                -1 if target_group is None else int(target_group))
+    r = lib().orc_compute_ordered(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(cl), nc, threads,
+                                  float("nan") if max_radius is None else float(max_radius), _p(g),
+                                  -1 if target_group is None else int(target_group), order)
     return _result(r, nc)
 
 

@@ -126,6 +126,7 @@ tess_status alloc_all(tess_diagram* d) {
             d->sorted.alloc(n) == hipSuccess && d->scratch.alloc(n) == hipSuccess &&
             d->sums.alloc((nb + 1023) / 1024 + 1) == hipSuccess && d->big.alloc(nb) == hipSuccess &&
             d->misc.alloc(4) == hipSuccess && d->sx.alloc(n) == hipSuccess && d->sy.alloc(n) == hipSuccess &&
+            (d->opts.point_order != TESS_ORDER_MORTON || d->codes.alloc(n) == hipSuccess) &&
             d->sz.alloc(n) == hipSuccess && d->volume.alloc(n) == hipSuccess &&
             d->fcount.alloc(n) == hipSuccess && d->status.alloc(n) == hipSuccess &&
             d->redo.alloc(n + 1) == hipSuccess && d->snbr.alloc(n * kFaceStage) == hipSuccess &&
@@ -148,7 +149,13 @@ tess_status bin_enqueue(tess_diagram* d) {
     return st;
   SortBufs b{d->bins.p, d->cnt.p, d->delim.p, d->cursor.p, d->sorted.p, d->scratch.p, d->sums.p, d->big.p,
              d->misc.p};
-  if ((st = c->hip(launch_counting_sort(d->x.p, d->y.p, d->z.p, n, d->grid.p, d->nbins, b, s), "sort")) !=
+  const uint32_t* key = nullptr;
+  if (d->opts.point_order == TESS_ORDER_MORTON) {  // a cell's points in the internal (Morton) order
+    if ((st = c->hip(launch_morton(d->x.p, d->y.p, d->z.p, n, d->grid.p, d->codes.p, s), "morton")) != TESS_OK)
+      return st;
+    key = d->codes.p;
+  }
+  if ((st = c->hip(launch_counting_sort(d->x.p, d->y.p, d->z.p, n, d->grid.p, d->nbins, b, s, key), "sort")) !=
       TESS_OK)
     return st;
   return c->hip(launch_gather(d->sorted.p, n, d->x.p, d->y.p, d->z.p, d->sx.p, d->sy.p, d->sz.p, s), "gather");
@@ -235,6 +242,10 @@ tess_status create_common(tess_ctx* ctx, size_t n, const double lo[3], const dou
   else
     tess_opts_default(&d->opts);
   if (!(d->opts.cell_density > 0)) d->opts.cell_density = 1.25;
+  if (d->opts.point_order != TESS_ORDER_INPUT && d->opts.point_order != TESS_ORDER_MORTON) {
+    delete d;
+    return ctx->fail(TESS_E_INVALID, "unknown point_order");
+  }
   d->n = n;
   d->cpd = icbrt_floor((double)n / d->opts.cell_density) + 1;  // celery.rs:161-162
   if (d->cpd > 1000) {
@@ -603,6 +614,38 @@ tess_status tess_diagram_copy_grid(const tess_diagram* d, uint32_t* cell_of_poin
     for (size_t i = 0; i < tmp.size(); ++i) delim[i] = tmp[i];
   }
   return d->ctx->hip(e, "download");
+}
+
+tess_status tess_diagram_copy_index_maps(tess_diagram* d, uint32_t* original_of_internal,
+                                         uint32_t* internal_of_original) {
+  if (!d) return TESS_E_INVALID;
+  tess_ctx* c = d->ctx;
+  std::lock_guard<std::mutex> g(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  tess_status st;
+  if (!d->binned && (st = bin(d)) != TESS_OK) return st;
+  const int n = (int)d->n;
+  DevBuf<uint32_t> codes, iota, keys, ooi, ioo;
+  DevBuf<uint8_t> tmp;
+  size_t tmp_bytes = 0;
+  if (codes.alloc(n) != hipSuccess || iota.alloc(n) != hipSuccess || keys.alloc(n) != hipSuccess ||
+      ooi.alloc(n) != hipSuccess || ioo.alloc(n) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if ((st = c->hip(launch_index_maps(codes.p, n, iota.p, keys.p, ooi.p, ioo.p, nullptr, &tmp_bytes, s), "sort")) !=
+      TESS_OK)
+    return st;
+  if (tmp.alloc(tmp_bytes) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if ((st = c->hip(launch_morton(d->x.p, d->y.p, d->z.p, n, d->grid.p, codes.p, s), "morton")) != TESS_OK ||
+      (st = c->hip(launch_index_maps(codes.p, n, iota.p, keys.p, ooi.p, ioo.p, tmp.p, &tmp_bytes, s), "sort")) !=
+          TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "index maps")) != TESS_OK)
+    return st;
+  hipError_t e = hipSuccess;
+  if (original_of_internal) e = hipMemcpy(original_of_internal, ooi.p, 4ull * n, hipMemcpyDeviceToHost);
+  if (internal_of_original && e == hipSuccess)
+    e = hipMemcpy(internal_of_original, ioo.p, 4ull * n, hipMemcpyDeviceToHost);
+  return c->hip(e, "download");
 }
 
 tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radius, int64_t target_group,

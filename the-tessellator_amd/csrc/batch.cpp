@@ -463,6 +463,7 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
   else
     tess_opts_default(&b->opts);
   if (!(b->opts.cell_density > 0)) b->opts.cell_density = 1.25;
+  if (b->opts.point_order != TESS_ORDER_INPUT) return ctx->fail(TESS_E_INVALID, "batches use the input point order");
   b->nsets = nsets;
   b->n.assign(n, n + nsets);
   size_t N = 0, B = 0, C = 0, S = 0, P = 0;

@@ -9,7 +9,8 @@
 //                (celery.rs:372-414)
 //   k_scatter    points to their cell's range (atomic cursor)
 //   k_sort_cells ascending point index inside each cell (stable order, the
-//                canonical form of celery.rs:357-369)
+//                canonical form of celery.rs:357-369), or ascending
+//                (Morton code, index) for the internal order
 //   k_gather     positions in grid order (coalesced candidate loads)
 //   k_compact    per-cell staged faces -> CSR
 // All passes are HBM-bound streams over the points or the cells.
@@ -233,9 +234,19 @@ __global__ void k_scatter(const uint32_t* __restrict__ bins, int n, uint32_t* __
 
 // Ascending point index inside each cell: insertion sort for small cells,
 // one wavefront per cell (rank by comparison) for large ones.
+// Order of two points of a cell: (key, index) when keys are given (the
+// internal Morton order), else the index.
+__device__ __forceinline__ bool before(const uint32_t* __restrict__ key, uint32_t a, uint32_t b) {
+  if (key) {
+    const uint32_t ka = key[a], kb = key[b];
+    if (ka != kb) return ka < kb;
+  }
+  return a < b;
+}
+
 __global__ void k_sort_cells(const uint32_t* __restrict__ delim, int nbins,
                              uint32_t* __restrict__ sorted, uint32_t* __restrict__ big,
-                             uint32_t* __restrict__ nbig) {
+                             uint32_t* __restrict__ nbig, const uint32_t* __restrict__ key) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   const uint32_t s = delim[b], e = delim[b + 1];
@@ -248,7 +259,7 @@ __global__ void k_sort_cells(const uint32_t* __restrict__ delim, int nbins,
   for (uint32_t i = s + 1; i < e; ++i) {
     const uint32_t v = sorted[i];
     uint32_t j = i;
-    while (j > s && sorted[j - 1] > v) {
+    while (j > s && before(key, v, sorted[j - 1])) {
       sorted[j] = sorted[j - 1];
       --j;
     }
@@ -262,7 +273,8 @@ __global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ d
                                                   const uint32_t* __restrict__ big,
                                                   const uint32_t* __restrict__ nbig,
                                                   uint32_t* __restrict__ sorted,
-                                                  uint32_t* __restrict__ scratch) {
+                                                  uint32_t* __restrict__ scratch,
+                                                  const uint32_t* __restrict__ key) {
   const uint32_t nb = *nbig;
   for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
     const uint32_t b = big[q];
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ d
     for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) {
       const uint32_t v = sorted[i];
       uint32_t r = 0;
-      for (uint32_t j = s; j < e; ++j) r += sorted[j] < v;
+      for (uint32_t j = s; j < e; ++j) r += before(key, sorted[j], v);
       scratch[s + r] = v;
     }
     __syncthreads();
@@ -396,7 +408,8 @@ hipError_t launch_bounds(const double* x, const double* y, const double* z, int 
 }
 
 hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
-                                const GridDev* g, int nbins, const SortBufs& b, hipStream_t s) {
+                                const GridDev* g, int nbins, const SortBufs& b, hipStream_t s,
+                                const uint32_t* key) {
   // fill kernels rather than memset nodes: this sequence may be captured
   // into a graph
   hipError_t e;
@@ -411,9 +424,9 @@ hipError_t launch_counting_sort(const double* x, const double* y, const double* 
   hipLaunchKernelGGL(k_copy_u32, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.cursor);
   hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, b.bins, n, b.cursor, b.sorted);
   hipLaunchKernelGGL(k_sort_cells, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sorted,
-                     b.big, b.misc + 1);
+                     b.big, b.misc + 1, key);
   hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(256), 0, s, b.delim, b.big, b.misc + 1, b.sorted,
-                     b.scratch);
+                     b.scratch, key);
   return hipGetLastError();
 }
 

@@ -20,8 +20,10 @@ struct SortBufs {
   uint32_t* big;      // [nbins]
   uint32_t* misc;     // [4]: total, nbig
 };
+// key (nullable): a cell's points ascend by (key[i], i) instead of i
 hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
-                                const GridDev* g, int nbins, const SortBufs& b, hipStream_t s);
+                                const GridDev* g, int nbins, const SortBufs& b, hipStream_t s,
+                                const uint32_t* key = nullptr);
 // The same binning without the in-cell ordering: b.sorted lists the n
 // positions grid cell by grid cell, in no set order inside a cell (only
 // bins, cnt, delim, cursor, sorted, sums and misc are used)
@@ -80,6 +82,16 @@ hipError_t launch_fill_u32(uint32_t* p, size_t m, uint32_t v, hipStream_t s);
 // offsets from launch_face_offsets over the counts)
 hipError_t launch_query_count(const QueryArgs& a, hipStream_t s);
 hipError_t launch_query_fill(const QueryArgs& a, hipStream_t s);
+
+// order.hip: Diagram::morton_index of every point (the box: the grid's
+// bounds grown to contain the origin) and the internal order it defines
+hipError_t launch_morton(const double* x, const double* y, const double* z, int n, const GridDev* g,
+                         uint32_t* codes, hipStream_t s);
+// original_of_internal = stable sort of 0..n-1 by code; internal_of_original
+// its inverse.  tmp: device scratch of *tmp_bytes (query with tmp = null).
+hipError_t launch_index_maps(const uint32_t* codes, int n, uint32_t* iota, uint32_t* keys_out,
+                             uint32_t* original_of_internal, uint32_t* internal_of_original, void* tmp,
+                             size_t* tmp_bytes, hipStream_t s);
 
 // mlp.hip: learned search-radius model (bf16 MFMA, fp32 accumulate)
 hipError_t launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);

@@ -140,6 +140,7 @@ struct tess_diagram {
   DevBuf<double> part;
   DevBuf<uint32_t> bins, cnt, delim, cursor, sorted, scratch, sums, big, misc, inv;
   DevBuf<double> sx, sy, sz;
+  DevBuf<uint32_t> codes;  // Morton codes (TESS_ORDER_MORTON)
   GridDev hgrid;
   bool binned = false;
   bool computed = false;

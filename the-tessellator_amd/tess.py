@@ -42,9 +42,10 @@ EXPORTS = [
     "tess_query_timings", "tess_query_destroy", "tess_diagram_cells_at_particles", "tess_site_cells_count",
     "tess_site_cells_total_faces", "tess_site_cells_copy", "tess_site_cells_copy_stats", "tess_site_cells_timings",
     "tess_site_cells_destroy", "tess_diagram_compute_geometry", "tess_diagram_geometry_sizes",
-    "tess_diagram_copy_geometry", "tess_diagram_geometry_timings",
+    "tess_diagram_copy_geometry", "tess_diagram_geometry_timings", "tess_diagram_copy_index_maps",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
+ORDER_INPUT, ORDER_MORTON = 0, 1
 MLP_IN, MLP_PARAMS = 32, 7312
 
 
@@ -56,7 +57,8 @@ class TessError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("cell_density", C.c_double), ("mode", C.c_int32), ("vertex_capacity", C.c_int32),
-                ("collect_stats", C.c_int32), ("xcd_chunk", C.c_int32)]
+                ("collect_stats", C.c_int32), ("xcd_chunk", C.c_int32), ("point_order", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class CellStats(C.Structure):
@@ -157,6 +159,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_diagram_geometry_sizes": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "tess_diagram_copy_geometry": (C.c_int, [vp, dp, dp, dp, dp]),
         "tess_diagram_geometry_timings": (C.c_int, [vp, dp]),
+        "tess_diagram_copy_index_maps": (C.c_int, [vp, dp, dp]),
         "tess_site_cells_count": (sz, [vp]),
         "tess_site_cells_total_faces": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
         "tess_site_cells_copy": (C.c_int, [vp, dp, dp, dp, dp, dp]),
@@ -340,7 +343,7 @@ class Diagram:
     """Points + container box (tess_diagram, interface.rs:24-84)."""
 
     def __init__(self, ctx: Context, xyz, lo, hi, groups=None, density=1.25, mode=MODE_PLAIN,
-                 vertex_capacity=0, collect_stats=False, xcd_chunk=0):
+                 vertex_capacity=0, collect_stats=False, xcd_chunk=0, point_order=ORDER_INPUT):
         self.ctx, self.lib = ctx, ctx.lib
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
@@ -348,6 +351,7 @@ class Diagram:
         self.lib.tess_opts_default(C.byref(o))
         o.cell_density, o.mode, o.vertex_capacity, o.collect_stats = density, mode, vertex_capacity, int(collect_stats)
         o.xcd_chunk = xcd_chunk
+        o.point_order = point_order
         h = C.c_void_p()
         if isinstance(xyz, DeviceArray):  # device-resident AoS points (and groups)
             assert xyz.dtype == np.float64 and xyz.nbytes % 24 == 0
@@ -409,6 +413,13 @@ class Diagram:
         b, s, cpd = np.empty(6), np.empty(3), C.c_int64()
         self.ctx.check(self.lib.tess_diagram_grid(self.h, _ptr(b), _ptr(s), C.byref(cpd)))
         return b, s, cpd.value
+
+    def index_maps(self):
+        """(original_of_internal, internal_of_original): the reference's
+        Morton internal order (sort_particles, interface.rs:88-148)."""
+        a, b = np.empty(self.n, np.uint32), np.empty(self.n, np.uint32)
+        self.ctx.check(self.lib.tess_diagram_copy_index_maps(self.h, _ptr(a), _ptr(b)))
+        return a, b
 
     def copy_grid(self):
         _, _, cpd = self.grid()
