@@ -211,7 +211,9 @@ size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
  * ring_offset has total_faces + 1 entries. */
 tess_status tess_diagram_compute_geometry(tess_diagram* d);
 tess_status tess_diagram_geometry_sizes(const tess_diagram* d, uint64_t* nvertices, uint64_t* nring);
-/* Device time of the last geometry call, ms: [0] count pass, [1] fill pass. */
+/* Device time of the last geometry call, ms: [0] the cell pass that stages
+ * the geometry (with its capacity rerun), [1] the offsets and the copy into
+ * the CSR. */
 tess_status tess_diagram_geometry_timings(const tess_diagram* d, double ms[2]);
 tess_status tess_diagram_copy_geometry(const tess_diagram* d, uint64_t* vertex_offset, double* vertices,
                                        uint64_t* ring_offset, uint16_t* ring);

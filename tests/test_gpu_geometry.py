@@ -88,7 +88,7 @@ def test_geometry_follows_recompute(ctx):
     with pytest.raises(tess.TessError):
         _geometry_sizes(d)
     g2 = d.geometry()
-    assert g2["timings"]["count_ms"] > 0 and g2["timings"]["fill_ms"] > 0
+    assert g2["timings"]["stage_ms"] > 0 and g2["timings"]["compact_ms"] > 0
     for k in ("vertex_offset", "vertices", "ring_offset", "ring"):
         assert np.array_equal(g1[k], g2[k]), k
 

@@ -152,7 +152,6 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ x,
 __global__ __launch_bounds__(256) void k_scan_block(const uint32_t* __restrict__ in, int m,
                                                     uint32_t* __restrict__ out,
                                                     uint32_t* __restrict__ sums) {
-  __shared__ uint32_t sm[1024];
   __shared__ uint32_t wsum[4];
   const int t = threadIdx.x, base = blockIdx.x * 1024;
   uint32_t v[4], s = 0;
@@ -181,7 +180,6 @@ __global__ __launch_bounds__(256) void k_scan_block(const uint32_t* __restrict__
     run += v[k];
   }
   if (t == 255) sums[blockIdx.x] = woff + incl;
-  (void)sm;
 }
 
 // Exclusive scan of the block sums in place (one block, sequential chunks).

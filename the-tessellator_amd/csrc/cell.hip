@@ -59,12 +59,6 @@ __device__ __forceinline__ double uni_d(double v) {
   int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-__device__ __forceinline__ double rl_d(double v, int l) {
-  unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
-  int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) {
     double t = __shfl_xor(v, o, 64);
@@ -699,7 +693,7 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
 // cell's status bits (wave-uniform); the caller queues capacity reruns.
 template <int VPL, int PPL, int FPL, bool GEO = false>
 __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos, Lds<VPL, PPL, FPL>& S,
-                                              const int lane) {
+                                              const int lane, const int slot = 0) {
   using L = Lds<VPL, PPL, FPL>;
   constexpr int FC = L::FC;
   CellState<VPL, PPL, FPL> C;
@@ -1166,62 +1160,59 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     }
     if constexpr (GEO) {
       // Geometry of whole sets (compute_vertices / compute_face_vertices,
-      // polyhedron.rs:885-919): the live vertices in slot order, and per face
-      // in output order the ring of local vertex numbers from the anchor (the
-      // lexicographically smallest vertex) along the face's orientation.
-      int nring = 0;
+      // polyhedron.rs:885-919), staged at this work item's slot: the live
+      // vertices in slot order, and per face in output order the ring of
+      // local vertex numbers from the anchor (the lexicographically smallest
+      // vertex) along the face's orientation, with the rings' offsets.
+      int below = 0;  // live slots in the lower registers
+      double* sv = A.geo_v + (size_t)slot * (3 * L::VC);
 #pragma unroll
-      for (int r = 0; r < FPL; ++r) nring += fal[r] ? ringk[r] : 0;
-      if (A.geo_count) {
-        nring = wave_incl_scan(nring, lane);
-        if (lane == 63) {
-          A.geo_count[oi] = (uint32_t)popc_multi(C.VA);
-          A.geo_count[(size_t)A.n + oi] = (uint32_t)nring;
+      for (int r = 0; r < VPL; ++r) {
+        if ((C.VA[r] >> lane) & 1ull) {
+          const int q = below + mbcnt(C.VA[r], lane);
+          const int v = lane + 64 * r;
+          sv[3 * q] = S.vx[v];
+          sv[3 * q + 1] = S.vy[v];
+          sv[3 * q + 2] = S.vz[v];
         }
-      } else {
-        const uint64_t vb = A.geo_voff[oi];
-        int below = 0;  // live slots in the lower registers
+        below += popc(C.VA[r]);
+      }
+      // ring offsets in output order: lengths by rank, then a wave scan
+      wave_sync();
 #pragma unroll
-        for (int r = 0; r < VPL; ++r) {
-          if ((C.VA[r] >> lane) & 1ull) {
-            const uint64_t q = vb + (uint64_t)(below + mbcnt(C.VA[r], lane));
-            const int v = lane + 64 * r;
-            A.geo_v[3 * q] = S.vx[v];
-            A.geo_v[3 * q + 1] = S.vy[v];
-            A.geo_v[3 * q + 2] = S.vz[v];
-          }
-          below += popc(C.VA[r]);
+      for (int r = 0; r < FPL; ++r)
+        if (fal[r]) S.bpref[rt[r]] = (uint32_t)ringk[r];
+      wave_sync();
+      const int kk = lane < nfaces ? (int)S.bpref[lane] : 0;
+      const int incl = wave_incl_scan(kk, lane);
+      const int ex = incl - kk;
+      if (lane < nfaces) A.geo_roff[(size_t)slot * kFaceStage + lane] = (uint16_t)ex;
+      S.bstart[lane] = (uint32_t)ex;
+      wave_sync();
+      uint16_t* sr = A.geo_ring + (size_t)slot * L::HC;
+#pragma unroll
+      for (int r = 0; r < FPL; ++r)
+        if (fal[r]) {
+          const int pos = (int)S.bstart[rt[r]];
+          int h = hanc[r], j = 0;
+          do {
+            const int v = S.he_tgt[h];
+            const int vr = v >> 6;
+            const uint64_t lower = (1ull << (v & 63)) - 1ull;
+            int rank = 0;  // live slots below v
+#pragma unroll
+            for (int q = 0; q < VPL; ++q)
+              rank += q < vr ? popc(C.VA[q]) : (q == vr ? popc(C.VA[q] & lower) : 0);
+            sr[pos + j] = (uint16_t)rank;
+            h = S.he_next[h];
+            ++j;
+          } while (h != hanc[r] && j < ringk[r]);
         }
-        // ring offsets in output order: lengths by rank, then a wave scan
-        wave_sync();
-#pragma unroll
-        for (int r = 0; r < FPL; ++r)
-          if (fal[r]) S.bpref[rt[r]] = (uint32_t)ringk[r];
-        wave_sync();
-        const int kk = lane < nfaces ? (int)S.bpref[lane] : 0;
-        const int ex = wave_incl_scan(kk, lane) - kk;
-        const uint64_t rb = A.geo_rbase[oi];
-        if (lane < nfaces) A.geo_roff[A.geo_foff[oi] + lane] = rb + (uint64_t)ex;
-        S.bstart[lane] = (uint32_t)ex;
-        wave_sync();
-#pragma unroll
-        for (int r = 0; r < FPL; ++r)
-          if (fal[r]) {
-            uint64_t pos = rb + S.bstart[rt[r]];
-            int h = hanc[r], j = 0;
-            do {
-              const int v = S.he_tgt[h];
-              const int vr = v >> 6;
-              const uint64_t lower = (1ull << (v & 63)) - 1ull;
-              int rank = 0;  // live slots below v
-#pragma unroll
-              for (int q = 0; q < VPL; ++q)
-                rank += q < vr ? popc(C.VA[q]) : (q == vr ? popc(C.VA[q] & lower) : 0);
-              A.geo_ring[pos + j] = (uint16_t)rank;
-              h = S.he_next[h];
-              ++j;
-            } while (h != hanc[r] && j < ringk[r]);
-          }
+      const int nring = rl_i(incl, 63);
+      if (lane == 0) {
+        A.geo_count[oi] = (uint32_t)below;
+        A.geo_count[(size_t)A.n + oi] = (uint32_t)nring;
+        A.geo_slot[slot] = oi + 1u;
       }
     }
   } else {
@@ -1270,7 +1261,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   const int w = A.xcd_chunk ? (int)xcd_map(blockIdx.x, (uint32_t)A.xcd_chunk) : (int)blockIdx.x;
   if (w >= A.nwork) return;
   const int gpos = A.work ? (int)A.work[w] : w;
-  const uint32_t status = cell_body<VPL, PPL, FPL, GEO>(A, gpos, S, lane);
+  const uint32_t status = cell_body<VPL, PPL, FPL, GEO>(A, gpos, S, lane, w);
   if (lane == 0 && (status & TESS_CELL_CAPACITY) && A.redo) {
     const uint32_t slot = atomicAdd(A.nredo, 1u);
     A.redo[slot] = (uint32_t)gpos;
@@ -1395,6 +1386,9 @@ hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
   }
   return hipGetLastError();
 }
+
+static_assert(Lds<1, 2, 1>::HC == kCellHalfEdges(1) && Lds<4, 6, 2>::HC == kCellHalfEdges(4),
+              "geometry staging capacity");
 
 hipError_t launch_cells_geometry(int vpl, const CellArgs& a, hipStream_t s) {
   if (a.nwork <= 0) return hipSuccess;

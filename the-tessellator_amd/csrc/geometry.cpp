@@ -21,12 +21,18 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   const GridDev& G = d->hgrid;
   Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
   if (!t) return st;
+  // One pass recomputes every cell like compute does (capacity 64, the
+  // overflows again at 256) and stages its geometry per work item; the
+  // counts give the CSR offsets and a compaction copies the staged data.
+  const size_t v1 = 64, h1 = kCellHalfEdges(1), v4 = 256, h4 = kCellHalfEdges(4);
+  DevBuf<uint32_t> slot1, slot4;
+  DevBuf<double> sv1, sv4;
+  DevBuf<uint16_t> sr1, sr4, sro1, sro4;
   if (d->geo_count.alloc(2 * n) != hipSuccess || d->geo_voff.alloc(n + 1) != hipSuccess ||
-      d->geo_rbase.alloc(n + 1) != hipSuccess || d->geo_sums.alloc((n + 1023) / 1024 + 1) != hipSuccess)
+      d->geo_rbase.alloc(n + 1) != hipSuccess || d->geo_sums.alloc((n + 1023) / 1024 + 1) != hipSuccess ||
+      slot1.alloc(n) != hipSuccess || sv1.alloc(3 * v1 * n) != hipSuccess || sr1.alloc(h1 * n) != hipSuccess ||
+      sro1.alloc((size_t)kFaceStage * n) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  // count pass: live vertices and ring entries per cell.  The cells are
-  // recomputed like compute does it: capacity 64 first, the overflows again
-  // at 256 (the geometry kernels write nothing else).
   CellArgs a = base_args(d, t);
   a.work = nullptr;
   a.nwork = (int)n;
@@ -36,35 +42,47 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   a.redo = d->redo.p + 1;
   a.nredo = d->redo.p;
   a.geo_count = d->geo_count.p;
-  CellArgs r = a;  // the rerun of the overflows
-  r.work = d->redo_list.p;
-  r.xcd_chunk = 0;
-  r.redo = nullptr;
-  r.nredo = nullptr;
-  uint32_t nredo = 0;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  a.geo_slot = slot1.p;
+  a.geo_v = sv1.p;
+  a.geo_ring = sr1.p;
+  a.geo_roff = sro1.p;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) return c->fail(TESS_E_DEVICE, "event create failed");
   struct Release {
     hipEvent_t* e;
     ~Release() {
-      for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e[k]);
+      for (int k = 0; k < 3; ++k) (void)hipEventDestroy(e[k]);
     }
   } release{ev};
+  uint32_t nredo = 0;
   if ((st = c->hip(launch_fill_u32(d->geo_count.p, 2 * n, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(slot1.p, n, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev[0], s), "event")) != TESS_OK ||
-      (st = c->hip(launch_cells_geometry(1, a, s), "geometry count")) != TESS_OK ||
+      (st = c->hip(launch_cells_geometry(1, a, s), "geometry")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "geometry count")) != TESS_OK)
+      (st = c->hip(hipStreamSynchronize(s), "geometry")) != TESS_OK)
     return st;
   if (nredo) {
-    if (d->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if (d->redo_list.alloc(nredo) != hipSuccess || slot4.alloc(nredo) != hipSuccess ||
+        sv4.alloc(3 * v4 * nredo) != hipSuccess || sr4.alloc(h4 * nredo) != hipSuccess ||
+        sro4.alloc((size_t)kFaceStage * nredo) != hipSuccess)
+      return c->fail(TESS_E_NOMEM, "device allocation failed");
+    CellArgs r = a;
     r.work = d->redo_list.p;
     r.nwork = (int)nredo;
+    r.xcd_chunk = 0;
+    r.redo = nullptr;
+    r.nredo = nullptr;
+    r.geo_slot = slot4.p;
+    r.geo_v = sv4.p;
+    r.geo_ring = sr4.p;
+    r.geo_roff = sro4.p;
     if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
                      "redo")) != TESS_OK ||
-        (st = c->hip(launch_cells_geometry(4, r, s), "geometry count")) != TESS_OK)
+        (st = c->hip(launch_fill_u32(slot4.p, nredo, 0u, s), "fill")) != TESS_OK ||
+        (st = c->hip(launch_cells_geometry(4, r, s), "geometry")) != TESS_OK)
       return st;
   }
   if ((st = c->hip(hipEventRecord(ev[1], s), "event")) != TESS_OK ||
@@ -76,35 +94,27 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
           TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&d->geo_nr, d->geo_rbase.p + n, 8, hipMemcpyDeviceToHost, s), "download")) !=
           TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "geometry count")) != TESS_OK)
+      (st = c->hip(hipStreamSynchronize(s), "geometry")) != TESS_OK)
     return st;
   const uint64_t F = d->total_faces;
   if (d->geo_v.alloc(3 * d->geo_nv) != hipSuccess || d->geo_roff.alloc(F + 1) != hipSuccess ||
       d->geo_ring.alloc(d->geo_nr) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  // fill pass at the offsets (the overflows write nothing at 64 and are
-  // filled by the rerun)
-  for (CellArgs* p : {&a, &r}) {
-    p->geo_count = nullptr;
-    p->geo_voff = d->geo_voff.p;
-    p->geo_foff = d->foff.p;
-    p->geo_rbase = d->geo_rbase.p;
-    p->geo_v = d->geo_v.p;
-    p->geo_roff = d->geo_roff.p;
-    p->geo_ring = d->geo_ring.p;
-  }
-  a.redo = nullptr;
-  a.nredo = nullptr;
   float ms[2] = {0, 0};
-  if ((st = c->hip(hipEventRecord(ev[2], s), "event")) != TESS_OK ||
-      (st = c->hip(launch_cells_geometry(1, a, s), "geometry fill")) != TESS_OK ||
-      (nredo && (st = c->hip(launch_cells_geometry(4, r, s), "geometry fill")) != TESS_OK) ||
-      (st = c->hip(hipEventRecord(ev[3], s), "event")) != TESS_OK ||
+  if ((st = c->hip(launch_geo_compact(slot1.p, (int)n, (int)v1, (int)h1, sv1.p, sr1.p, sro1.p, d->geo_count.p,
+                                      (int)n, d->geo_voff.p, d->geo_rbase.p, d->foff.p, d->geo_v.p, d->geo_ring.p,
+                                      d->geo_roff.p, s),
+                   "compact")) != TESS_OK ||
+      (nredo && (st = c->hip(launch_geo_compact(slot4.p, (int)nredo, (int)v4, (int)h4, sv4.p, sr4.p, sro4.p,
+                                                d->geo_count.p, (int)n, d->geo_voff.p, d->geo_rbase.p, d->foff.p,
+                                                d->geo_v.p, d->geo_ring.p, d->geo_roff.p, s),
+                             "compact")) != TESS_OK) ||
       (st = c->hip(hipMemcpyAsync(d->geo_roff.p + F, &d->geo_nr, 8, hipMemcpyHostToDevice, s), "upload")) !=
           TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "geometry fill")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev[2], s), "event")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "geometry")) != TESS_OK ||
       (st = c->hip(hipEventElapsedTime(&ms[0], ev[0], ev[1]), "event")) != TESS_OK ||
-      (st = c->hip(hipEventElapsedTime(&ms[1], ev[2], ev[3]), "event")) != TESS_OK)
+      (st = c->hip(hipEventElapsedTime(&ms[1], ev[1], ev[2]), "event")) != TESS_OK)
     return st;
   d->geo_ms[0] = ms[0];
   d->geo_ms[1] = ms[1];

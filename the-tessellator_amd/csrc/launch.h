@@ -43,8 +43,15 @@ hipError_t launch_invert(const uint32_t* sorted, int n, uint32_t* inv, hipStream
 // cell.hip: vpl = vertex slots per lane (1, 2 or 4)
 hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s);
 // the same cells with the geometry export only (CellArgs::geo_*; the other
-// outputs are not written), vpl 1 or 4
+// outputs are not written), vpl 1 or 4; staging capacity per work item:
+// 64 vpl vertices, kCellHalfEdges(vpl) ring entries, kFaceStage faces
 hipError_t launch_cells_geometry(int vpl, const CellArgs& a, hipStream_t s);
+constexpr int kCellHalfEdges(int vpl) { return vpl == 1 ? 256 : 768; }
+// geometry.hip: staged geometry of nslots work items -> the CSR
+hipError_t launch_geo_compact(const uint32_t* slot_cell, int nslots, int vcap, int hcap, const double* sv,
+                              const uint16_t* sr, const uint16_t* sro, const uint32_t* count, int n,
+                              const uint64_t* voff, const uint64_t* rbase, const uint64_t* foff, double* out_v,
+                              uint16_t* out_ring, uint64_t* out_roff, hipStream_t s);
 // many sets: one wavefront per item of a list of host-known length n ...
 hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream_t s);
 // ... or persistent over a device-counted list at the largest capacity

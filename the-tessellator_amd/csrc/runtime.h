@@ -165,7 +165,7 @@ struct tess_diagram {
   DevBuf<double> geo_v;
   DevBuf<uint16_t> geo_ring;
   uint64_t geo_nv = 0, geo_nr = 0;
-  double geo_ms[2] = {0, 0};  // count pass, fill pass
+  double geo_ms[2] = {0, 0};  // staging pass (+ rerun), scans + compaction
   bool geo_ready = false;
   uint64_t total_faces = 0;
   tess_timings tim;

@@ -85,18 +85,16 @@ struct CellArgs {
   Summary* summary;        // nullable
   uint32_t* redo;          // grid positions of cells needing a larger variant
   uint32_t* nredo;
-  // geometry of whole sets (the geometry kernel only): the count pass writes
-  // geo_count[o] = live vertices, [n + o] = ring entries of cell o; the
-  // fill pass writes vertices from geo_voff[o], ring offsets of the cell's
-  // faces at geo_roff[geo_foff[o] ...] and ring entries (vertex numbers
-  // within the cell) from geo_rbase[o]
+  // geometry of whole sets (the geometry kernel only), staged per work item
+  // w: vertices geo_v[w][64 VPL][3], rings geo_ring[w][half-edge capacity],
+  // ring offsets in the cell geo_roff[w][kFaceStage]; geo_count[o] = live
+  // vertices and geo_count[n + o] = ring entries of cell o; geo_slot[w] =
+  // o + 1 once staged (0: nothing staged, e.g. a capacity overflow)
   uint32_t* geo_count;
-  const uint64_t* geo_voff;
-  const uint64_t* geo_foff;
-  const uint64_t* geo_rbase;
+  uint32_t* geo_slot;
   double* geo_v;
-  uint64_t* geo_roff;
   uint16_t* geo_ring;
+  uint16_t* geo_roff;
   // single-cell geometry export (nullable): vertices and face rings
   double* exp_v;           // [256][3]
   int32_t* exp_ring;       // [256 faces][64] vertex ids, -1 terminated

@@ -510,7 +510,7 @@ class Diagram:
         ms = np.zeros(2)
         self.ctx.check(self.lib.tess_diagram_geometry_timings(self.h, _ptr(ms)))
         return dict(vertex_offset=vo, vertices=v, ring_offset=ro, ring=ring,
-                    timings=dict(count_ms=float(ms[0]), fill_ms=float(ms[1])))
+                    timings=dict(stage_ms=float(ms[0]), compact_ms=float(ms[1])))
 
     def cells_at_particles(self, sites, search_radius=None, target_group=None, stats=False):
         """Diagram::get_cell_at_particle + compute_voronoi_cell for sites that
