@@ -285,8 +285,10 @@ tess_status tess_query_copy(const tess_query* q, uint64_t* offsets, uint32_t* id
 /* Device pointers owned by q (valid until tess_query_destroy). */
 tess_status tess_query_device(const tess_query* q, const uint64_t** offsets, const uint32_t** ids);
 /* Device time, ms: [0] the count pass, [1] the fill pass, [2] the whole
- * call from the count pass to the end of the fill pass (with the scan and
- * the sizing of the id buffer between them). */
+ * call from the count pass to the last kernel (with the scans, the sizing
+ * of the id buffer and, for the real radius and grouped clouds, the packing
+ * of the ids the fill pass wrote into upper-bound slots: for those kinds the
+ * count pass counts the unfiltered walk from the delimiters alone). */
 tess_status tess_query_timings(const tess_query* q, double ms[3]);
 void tess_query_destroy(tess_query* q);
 

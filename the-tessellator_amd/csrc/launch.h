@@ -86,9 +86,13 @@ hipError_t launch_mark_status(const uint32_t* list, int m, const uint32_t* sorte
 hipError_t launch_fill_u32(uint32_t* p, size_t m, uint32_t v, hipStream_t s);
 
 // query.hip: radius queries, one wavefront per query (count, then fill at
-// offsets from launch_face_offsets over the counts)
+// offsets from launch_face_offsets over the counts; a fill with count set
+// also writes the ids it kept per query, for fills into upper-bound slots
+// that launch_query_compact then packs)
 hipError_t launch_query_count(const QueryArgs& a, hipStream_t s);
 hipError_t launch_query_fill(const QueryArgs& a, hipStream_t s);
+hipError_t launch_query_compact(int nq, const uint64_t* from, const uint64_t* to, const uint32_t* src, uint32_t* dst,
+                                hipStream_t s);
 
 // order.hip: Diagram::morton_index of every point (the box: the grid's
 // bounds grown to contain the origin) and the internal order it defines

@@ -41,8 +41,12 @@ int cloud_table_extent(int64_t cpd, const GridDev& G, const double* r, size_t m)
 // Query sets at least this large are taken in grid order.
 constexpr size_t kQueryOrderMin = 4096;
 
+// Largest upper-bound slot buffer of a filtered query set (8 GiB of ids);
+// larger sets count exactly first.
+constexpr uint64_t kBoundMax = 1ull << 31;
+
 struct Events {
-  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   Events() {
     for (auto& x : e) (void)hipEventCreate(&x);
   }
@@ -149,6 +153,54 @@ tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, 
   a.tlen = t ? t->len : 0;
   a.count = cnt.p;
   Events ev;
+  // Filtered kinds (real radius, grouped clouds): the unfiltered walk gives
+  // upper bounds from the delimiters alone, one fill pass writes the kept
+  // ids into the bound slots and counts them, and a compaction packs them:
+  // the points are read once instead of once per pass.
+  const bool filtered = kind == TESS_QUERY_REAL_RADIUS || (kind == TESS_QUERY_CLOUD && target_group >= 0);
+  if (filtered) {
+    QueryArgs b = a;
+    b.kind = kind == TESS_QUERY_REAL_RADIUS ? TESS_QUERY_CELL_RADIUS : TESS_QUERY_CLOUD;
+    b.groups = nullptr;
+    DevBuf<uint64_t> boff;
+    uint64_t bound = 0;
+    if (boff.alloc(nq + 1) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if ((st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
+        (st = c->hip(launch_query_count(b, s), "query bound")) != TESS_OK ||
+        (st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
+        (st = c->hip(launch_face_offsets(cnt.p, (int)nq, boff.p, sums.p, s), "query scan")) != TESS_OK ||
+        (st = c->hip(hipMemcpyAsync(&bound, boff.p + nq, 8, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
+        (st = c->hip(hipStreamSynchronize(s), "query bound")) != TESS_OK)
+      return st;
+    DevBuf<uint32_t> slots;
+    if (bound <= kBoundMax && slots.alloc(bound) == hipSuccess) {
+      a.offsets = boff.p;
+      a.ids = slots.p;
+      a.count = cnt.p;  // kept per query
+      if ((st = c->hip(hipEventRecord(ev.e[2], s), "event")) != TESS_OK ||
+          (st = c->hip(launch_query_fill(a, s), "query fill")) != TESS_OK ||
+          (st = c->hip(hipEventRecord(ev.e[3], s), "event")) != TESS_OK ||
+          (st = c->hip(launch_face_offsets(cnt.p, (int)nq, q->offsets.p, sums.p, s), "query scan")) != TESS_OK ||
+          (st = c->hip(hipMemcpyAsync(&q->total, q->offsets.p + nq, 8, hipMemcpyDeviceToHost, s), "download")) !=
+              TESS_OK ||
+          (st = c->hip(hipStreamSynchronize(s), "query fill")) != TESS_OK)
+        return st;
+      if (q->ids.alloc(q->total) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+      float ms[3] = {0, 0, 0};
+      if ((st = c->hip(launch_query_compact((int)nq, boff.p, q->offsets.p, slots.p, q->ids.p, s), "compact")) !=
+              TESS_OK ||
+          (st = c->hip(hipEventRecord(ev.e[4], s), "event")) != TESS_OK ||
+          (st = c->hip(hipStreamSynchronize(s), "query compact")) != TESS_OK ||
+          (st = c->hip(hipEventElapsedTime(&ms[0], ev.e[0], ev.e[1]), "event")) != TESS_OK ||
+          (st = c->hip(hipEventElapsedTime(&ms[1], ev.e[2], ev.e[3]), "event")) != TESS_OK ||
+          (st = c->hip(hipEventElapsedTime(&ms[2], ev.e[0], ev.e[4]), "event")) != TESS_OK)
+        return st;
+      for (int k = 0; k < 3; ++k) q->ms[k] = ms[k];
+      *out = q.release();
+      return TESS_OK;
+    }
+    a.count = cnt.p;  // too many slots: count exactly, then fill
+  }
   if ((st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
       (st = c->hip(launch_query_count(a, s), "query count")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
@@ -160,6 +212,7 @@ tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, 
   if (q->ids.alloc(q->total) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
   a.offsets = q->offsets.p;
   a.ids = q->ids.p;
+  a.count = nullptr;
   float ms[3] = {0, 0, 0};
   if ((st = c->hip(hipEventRecord(ev.e[2], s), "event")) != TESS_OK ||
       (st = c->hip(launch_query_fill(a, s), "query fill")) != TESS_OK ||

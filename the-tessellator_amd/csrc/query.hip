@@ -130,7 +130,8 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
   const double r2 = r * r;
   const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
   const uint64_t njk = (uint64_t)(nj > 0 ? nj : 0) * (uint64_t)(nk > 0 ? nk : 0);
-  const uint64_t total = ni > 0 ? (uint64_t)ni * njk : 0;
+  // a NaN radius admits no cell (ds <= NaN is false): skip the walk
+  const uint64_t total = (ni > 0 && r == r) ? (uint64_t)ni * njk : 0;
   uint64_t cursor = FILL ? a.offsets[qi] : 0;
   const uint64_t end = FILL ? a.offsets[qi + 1] : 0;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
       emit_points<KIND == 1 ? 0 : 1, FILL>(a, s0, len, lane, qx, qy, qz, r2, cursor, end);
     }
   }
-  if (!FILL && lane == 0) a.count[qi] = (uint32_t)cursor;
+  if (lane == 0 && a.count) a.count[qi] = (uint32_t)(FILL ? cursor - a.offsets[qi] : cursor);
 }
 
 // Cell::compute_neighbor_cloud: expand_all_in_radius from the query's grid
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(256) void k_query_cloud(QueryArgs a) {
       emit_points<0, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end);
     if (__ballot(beyond)) break;
   }
-  if (!FILL && lane == 0) a.count[qi] = (uint32_t)cursor;
+  if (lane == 0 && a.count) a.count[qi] = (uint32_t)(FILL ? cursor - a.offsets[qi] : cursor);
 }
 
 template <bool FILL>
@@ -232,7 +233,26 @@ hipError_t launch_query_pass(const QueryArgs& a, hipStream_t s) {
 
 }  // namespace
 
+// Ids of query q from src[from[q] ..] to dst[to[q] .. to[q + 1]), one
+// wavefront per query.
+__global__ __launch_bounds__(256) void k_query_compact(int nq, const uint64_t* __restrict__ from,
+                                                       const uint64_t* __restrict__ to,
+                                                       const uint32_t* __restrict__ src,
+                                                       uint32_t* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  const uint64_t f = from[q], t = to[q], m = to[q + 1] - t;
+  for (uint64_t k = lane; k < m; k += 64) dst[t + k] = src[f + k];
+}
+
 hipError_t launch_query_count(const QueryArgs& a, hipStream_t s) { return launch_query_pass<false>(a, s); }
 hipError_t launch_query_fill(const QueryArgs& a, hipStream_t s) { return launch_query_pass<true>(a, s); }
+hipError_t launch_query_compact(int nq, const uint64_t* from, const uint64_t* to, const uint32_t* src, uint32_t* dst,
+                                hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query_compact, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, nq, from, to, src, dst);
+  return hipGetLastError();
+}
 
 }  // namespace tess
