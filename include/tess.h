@@ -184,6 +184,11 @@ tess_status tess_diagram_copy_index_maps(tess_diagram* d, uint32_t* original_of_
  * target_group: < 0 for None. */
 tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radius,
                                int64_t target_group, tess_cell** out);
+/* Diagram::get_cell_at_particle (interface.rs:211-232): the cell of a site
+ * that is not a point of the diagram (inside the container box); no point
+ * is excluded as its own, and tess_cell_original_index returns -1 (None). */
+tess_status tess_cell_at_particle(tess_diagram* d, const double xyz[3], double search_radius,
+                                  int64_t target_group, tess_cell** out);
 void tess_cell_destroy(tess_cell* c);
 int64_t tess_cell_original_index(const tess_cell* c);             /* interface.rs:387-389 */
 double tess_cell_volume(const tess_cell* c);                      /* interface.rs:337-339 */

@@ -91,3 +91,34 @@ def test_site_errors(ctx):
     d2 = tess.Diagram(ctx, pts, LO, HI, density=5.0)
     with pytest.raises(tess.TessError):
         d2.cells_at_particles([[0.5, 0.5, 0.5]], stats=True)  # created without collect_stats
+
+
+def test_cell_at_particle_single(ctx):
+    """tess_cell_at_particle: the Cell API for one outside site equals the
+    batched result (and the oracle's), and leaves the diagram's cells alone."""
+    pts = tess.clustered(20000, 69)
+    sites = _sites(pts, 70, 40)
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0)
+    d.compute()
+    before = d.cells()
+    batch = d.cells_at_particles(sites)
+    want = orc.compute_at_points(pts, sites, LO, HI, density=5.0)
+    fo = batch["face_offset"]
+    for i in range(len(sites)):
+        c = d.get_cell_at_particle(sites[i])
+        assert c.original_index() == -1
+        assert c.compute_volume() == batch["volume"][i] == want["volume"][i]
+        nb = c.compute_neighbors()
+        assert np.array_equal(nb, batch["neighbor"][fo[i]:fo[i + 1]])
+        for f in range(c.face_count()):
+            area, _ = c.face(f)
+            ring = c.face_vertices(f)
+            w = np.zeros(3)
+            for k in range(1, len(ring) - 1):
+                w += np.cross(ring[k] - ring[0], ring[k + 1] - ring[0])
+            assert abs(0.5 * np.linalg.norm(w) - area) <= 1e-12 * max(area, 1e-12)
+    after = d.cells()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    with pytest.raises(tess.TessError):
+        d.get_cell_at_particle([2.0, 0.5, 0.5])

@@ -648,12 +648,23 @@ tess_status tess_diagram_copy_index_maps(tess_diagram* d, uint32_t* original_of_
   return c->hip(e, "download");
 }
 
-tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radius, int64_t target_group,
-                               tess_cell** out) {
+}  // extern "C"
+
+namespace {
+
+// One cell with its geometry through the single-cell export of the cell
+// kernel: the point `index` of the diagram, or the outside site `site`
+// (host, 3 doubles; index ignored).  Capacity 128 first, then 256.
+tess_status single_cell(tess_diagram* d, size_t index, const double* site, double search_radius,
+                        int64_t target_group, tess_cell** out) {
   if (!d || !out) return TESS_E_INVALID;
   *out = nullptr;
   tess_ctx* c = d->ctx;
-  if (index >= d->n) return c->fail(TESS_E_INVALID, "index out of range");
+  if (!site && index >= d->n) return c->fail(TESS_E_INVALID, "index out of range");
+  if (site)
+    for (int k = 0; k < 3; ++k)
+      if (!(site[k] >= d->lo[k] && site[k] <= d->hi[k]))
+        return c->fail(TESS_E_INVALID, "the site is outside the container box");
   if (target_group >= 0 && !d->have_groups) return c->fail(TESS_E_INVALID, "diagram has no groups");
   std::lock_guard<std::mutex> g(c->mu);
   (void)hipSetDevice(c->device);
@@ -663,7 +674,7 @@ tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radi
     if ((st = bin(d)) != TESS_OK) return st;
     if ((st = c->hip(hipStreamSynchronize(s), "binning")) != TESS_OK) return st;
   }
-  if (!d->inv_ready) {
+  if (!site && !d->inv_ready) {
     if (d->inv.alloc(d->n) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
     if ((st = c->hip(launch_invert(d->sorted.p, (int)d->n, d->inv.p, s), "invert")) != TESS_OK) return st;
     d->inv_ready = true;
@@ -671,57 +682,69 @@ tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radi
   if (d->exp_v.alloc(3 * 256) != hipSuccess || d->exp_meta.alloc(2 + 256) != hipSuccess ||
       d->exp_ring.alloc(kFaceStage * 64) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if (site && (d->one_site.alloc(3) != hipSuccess || d->one_vol.alloc(1) != hipSuccess ||
+               d->one_u32.alloc(2) != hipSuccess || d->one_nbr.alloc(kFaceStage) != hipSuccess ||
+               d->one_area.alloc(kFaceStage) != hipSuccess))
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
   const GridDev& G = d->hgrid;
   Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
   if (!t) return st;
-  const uint32_t oi = (uint32_t)index;
+  // outputs: the diagram's slot of the point, or the single-site buffers
+  const uint32_t oi = site ? 0u : (uint32_t)index;
+  double* vol = site ? d->one_vol.p : d->volume.p;
+  uint32_t* fcount = site ? d->one_u32.p : d->fcount.p;
+  uint32_t* status_p = site ? d->one_u32.p + 1 : d->status.p;
+  int32_t* snbr = site ? d->one_nbr.p : d->snbr.p;
+  double* sarea = site ? d->one_area.p : d->sarea.p;
+  if (site && (st = c->hip(hipMemcpyAsync(d->one_site.p, site, 24, hipMemcpyHostToDevice, s), "upload")) != TESS_OK)
+    return st;
   uint32_t tries[2] = {2, 4};
-  tess_cell* cell = new (std::nothrow) tess_cell;
+  std::unique_ptr<tess_cell> cell(new (std::nothrow) tess_cell);
   if (!cell) return c->fail(TESS_E_NOMEM, "host allocation failed");
   uint32_t status = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
     CellArgs a = base_args(d, t);
-    a.work = d->work1.p;
+    a.work = site ? nullptr : d->work1.p;
+    a.ext = site ? d->one_site.p : nullptr;
     a.nwork = 1;
     a.xcd_chunk = 0;
     a.target_group = target_group;
     a.radius_limit = search_radius;
+    a.volume = vol;
+    a.fcount = fcount;
+    a.status = status_p;
+    a.snbr = snbr;
+    a.sarea = sarea;
+    a.stats = site ? nullptr : a.stats;
     a.exp_v = d->exp_v.p;
     a.exp_ring = d->exp_ring.p;
     a.exp_meta = d->exp_meta.p;
     a.redo = nullptr;
     a.nredo = nullptr;
-    if ((st = c->hip(hipMemcpyAsync(d->work1.p, d->inv.p + oi, 4, hipMemcpyDeviceToDevice, s), "copy")) !=
-            TESS_OK ||
+    if ((!site && (st = c->hip(hipMemcpyAsync(d->work1.p, d->inv.p + oi, 4, hipMemcpyDeviceToDevice, s), "copy")) !=
+                      TESS_OK) ||
         (st = c->hip(launch_cells((int)tries[attempt], a, s), "cell kernel")) != TESS_OK ||
-        (st = c->hip(hipMemcpyAsync(&status, d->status.p + oi, 4, hipMemcpyDeviceToHost, s), "copy")) !=
-            TESS_OK ||
-        (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK) {
-      delete cell;
+        (st = c->hip(hipMemcpyAsync(&status, status_p + oi, 4, hipMemcpyDeviceToHost, s), "copy")) != TESS_OK ||
+        (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
       return st;
-    }
     if (!(status & TESS_CELL_CAPACITY)) break;
   }
-  if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_TABLE)) {
-    delete cell;
+  if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_TABLE))
     return c->fail(status & TESS_CELL_DEGENERATE ? TESS_E_DEGENERATE : TESS_E_CAPACITY,
-                   "cell " + std::to_string(index) + " failed (status " + std::to_string(status) + ")");
-  }
+                   (site ? std::string("the site's cell") : "cell " + std::to_string(index)) + " failed (status " +
+                       std::to_string(status) + ")");
   uint32_t fc = 0;
   std::vector<int32_t> nb(kFaceStage), ring(kFaceStage * 64), meta(2 + 256);
   std::vector<double> ar(kFaceStage), vv(3 * 256);
-  hipError_t e = hipMemcpy(&cell->volume, d->volume.p + oi, 8, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(&fc, d->fcount.p + oi, 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(nb.data(), d->snbr.p + (size_t)oi * kFaceStage, 4 * kFaceStage, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(ar.data(), d->sarea.p + (size_t)oi * kFaceStage, 8 * kFaceStage, hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpy(&cell->volume, vol + oi, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&fc, fcount + oi, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nb.data(), snbr + (size_t)oi * kFaceStage, 4 * kFaceStage, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ar.data(), sarea + (size_t)oi * kFaceStage, 8 * kFaceStage, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(ring.data(), d->exp_ring.p, 4 * ring.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(meta.data(), d->exp_meta.p, 4 * meta.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(vv.data(), d->exp_v.p, 8 * vv.size(), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    delete cell;
-    return c->hip(e, "download");
-  }
-  cell->original = (int64_t)index;
+  if (e != hipSuccess) return c->hip(e, "download");
+  cell->original = site ? -1 : (int64_t)index;  // Cell::original_index: None for a site
   const int vc = meta[0];
   for (uint32_t f = 0; f < fc; ++f) {
     cell->nbr.push_back(nb[f]);
@@ -741,8 +764,23 @@ tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radi
       cell->verts.push_back(vv[3 * v + 1]);
       cell->verts.push_back(vv[3 * v + 2]);
     }
-  *out = cell;
+  *out = cell.release();
   return TESS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tess_status tess_cell_at_index(tess_diagram* d, size_t index, double search_radius, int64_t target_group,
+                               tess_cell** out) {
+  return single_cell(d, index, nullptr, search_radius, target_group, out);
+}
+
+tess_status tess_cell_at_particle(tess_diagram* d, const double xyz[3], double search_radius, int64_t target_group,
+                                  tess_cell** out) {
+  if (!xyz) return d ? d->ctx->fail(TESS_E_INVALID, "null site") : TESS_E_INVALID;
+  return single_cell(d, 0, xyz, search_radius, target_group, out);
 }
 
 void tess_cell_destroy(tess_cell* c) { delete c; }

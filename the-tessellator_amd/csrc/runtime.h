@@ -159,6 +159,10 @@ struct tess_diagram {
   DevBuf<uint32_t> sched_key, sched_list, sched_hist, sched_sums;
   DevBuf<double> exp_v;
   DevBuf<int32_t> exp_ring, exp_meta;
+  // outputs of one outside site (tess_cell_at_particle)
+  DevBuf<double> one_site, one_vol, one_area;
+  DevBuf<uint32_t> one_u32;  // face count, status
+  DevBuf<int32_t> one_nbr;
   // geometry of every cell (tess_diagram_compute_geometry)
   DevBuf<uint32_t> geo_count;
   DevBuf<uint64_t> geo_voff, geo_rbase, geo_roff, geo_sums;

@@ -43,6 +43,7 @@ EXPORTS = [
     "tess_site_cells_total_faces", "tess_site_cells_copy", "tess_site_cells_copy_stats", "tess_site_cells_timings",
     "tess_site_cells_destroy", "tess_diagram_compute_geometry", "tess_diagram_geometry_sizes",
     "tess_diagram_copy_geometry", "tess_diagram_geometry_timings", "tess_diagram_copy_index_maps",
+    "tess_cell_at_particle",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 ORDER_INPUT, ORDER_MORTON = 0, 1
@@ -117,6 +118,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_diagram_grid": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int64)]),
         "tess_diagram_copy_grid": (C.c_int, [vp, dp, dp, dp]),
         "tess_cell_at_index": (C.c_int, [vp, sz, C.c_double, C.c_int64, C.POINTER(vp)]),
+        "tess_cell_at_particle": (C.c_int, [vp, dp, C.c_double, C.c_int64, C.POINTER(vp)]),
         "tess_cell_destroy": (None, [vp]),
         "tess_cell_original_index": (C.c_int64, [vp]),
         "tess_cell_volume": (C.c_double, [vp]),
@@ -538,6 +540,16 @@ class Diagram:
         finally:
             self.lib.tess_site_cells_destroy(h)
         return out
+
+    def get_cell_at_particle(self, xyz, search_radius=None, target_group=None):
+        """interface.rs:211-232 + Cell::compute_voronoi_cell for a site that
+        is not a point of the diagram."""
+        h = C.c_void_p()
+        p = np.ascontiguousarray(xyz, np.float64).reshape(3)
+        r = math.nan if search_radius is None else float(search_radius)
+        g = -1 if target_group is None else int(target_group)
+        self.ctx.check(self.lib.tess_cell_at_particle(self.h, _ptr(p), r, g, C.byref(h)))
+        return Cell(self.lib, h)
 
     def get_cell_at_index(self, index, search_radius=None, target_group=None):
         """interface.rs:186-208 + Cell::compute_voronoi_cell."""
