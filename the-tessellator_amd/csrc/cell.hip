@@ -59,22 +59,46 @@ __device__ __forceinline__ double uni_d(double v) {
   int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    double t = __shfl_xor(v, o, 64);
-    v = t > v ? t : v;
-  }
-  return v;
+// v of the lane the DPP control names (the lane's own v where it has none
+// or its row is masked off).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = (int)(u & 0xffffffffu), hi = (int)(u >> 32);
+  const int slo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const int shi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)shi << 32) | (unsigned)slo));
+}
+// Maximum over the wavefront (max is order-free), scalar: DPP row shifts
+// inside the rows of 16, the row maxima broadcast to the later rows, lane 63.
+__device__ __forceinline__ double wave_max_uni(double v) {
+  double t;
+  t = dpp_d<0x111, 0xf>(v); v = t > v ? t : v;  // row_shr:1
+  t = dpp_d<0x112, 0xf>(v); v = t > v ? t : v;  // row_shr:2
+  t = dpp_d<0x114, 0xf>(v); v = t > v ? t : v;  // row_shr:4
+  t = dpp_d<0x118, 0xf>(v); v = t > v ? t : v;  // row_shr:8
+  t = dpp_d<0x142, 0xa>(v); v = t > v ? t : v;  // row_bcast:15 -> rows 1, 3
+  t = dpp_d<0x143, 0xc>(v); v = t > v ? t : v;  // row_bcast:31 -> rows 2, 3
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Inclusive prefix sum over the wavefront with DPP row shifts inside each
+// row of 16 lanes, then the row totals broadcast by row_bcast:15 / :31
+// (lanes without a source add 0).
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-  for (int o = 1; o < 64; o <<= 1) {
-    int t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
+  (void)lane;
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 -> rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 -> rows 2, 3
   return v;
 }
 
@@ -663,7 +687,7 @@ __device__ double cell_r2(const CellState<VPL, PPL, FPL>& C, int lane) {
       double d = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
       m = d > m ? d : m;
     }
-  return uni_d(wave_max(m));  // every lane holds the maximum; keep it scalar
+  return wave_max_uni(m);
 }
 
 // cell_r2 and a vertex slot that attains it.
@@ -676,7 +700,7 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
     d[r] = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
     if ((C.VA[r] >> lane) & 1ull) m = d[r] > m ? d[r] : m;
   }
-  m = uni_d(wave_max(m));
+  m = wave_max_uni(m);
   int s = -1;
 #pragma unroll
   for (int r = VPL - 1; r >= 0; --r) {

@@ -38,16 +38,20 @@ __device__ __forceinline__ int cell_gap(int a, int b) {
   return d > 0 ? d : 0;
 }
 
-// Exclusive prefix of v over the wavefront; *total = the sum.
+// Exclusive prefix of v over the wavefront; *total = the sum.  DPP row
+// shifts inside each row of 16 lanes, then the row totals broadcast by
+// row_bcast:15 / :31 (lanes without a source add 0).
 __device__ __forceinline__ uint32_t wave_excl(uint32_t v, int lane, uint32_t* total) {
-  uint32_t incl = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += u;
-  }
-  *total = __shfl(incl, 63, 64);
-  return incl - v;
+  (void)lane;
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 -> rows 1, 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 -> rows 2, 3
+  *total = (uint32_t)__builtin_amdgcn_readlane(x, 63);
+  return (uint32_t)x - v;
 }
 
 // The points of the step's 64 grid cells, in cell order and, inside a cell,
