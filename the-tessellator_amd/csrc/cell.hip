@@ -88,6 +88,18 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Running maximum over the lanes (inclusive), DPP as wave_incl_scan; lanes
+// without a source contribute -1 (values are >= -1).
+__device__ __forceinline__ int wave_max_scan(int v) {
+  int t;
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:1
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:2
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:4
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:8
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false); v = t > v ? t : v;  // row_bcast:15
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false); v = t > v ? t : v;  // row_bcast:31
+  return v;
+}
 // Inclusive prefix sum over the wavefront with DPP row shifts inside each
 // row of 16 lanes, then the row totals broadcast by row_bcast:15 / :31
 // (lanes without a source add 0).
@@ -225,6 +237,7 @@ struct Lds {
   int32_t f_tag[FC];
   uint32_t bstart[64];
   uint32_t bpref[64];
+  uint8_t head[64];  // streaming: the entry whose candidates start at chunk position p (0xff: none)
   uint16_t he_next[HC];
   uint16_t f_start[FC];
   uint16_t f_seq[FC];
@@ -844,25 +857,23 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       const int incl = wave_incl_scan(c, lane);
       const int total = rl_i(incl, 63);
       if (total == 0) break;
-      S.bpref[lane] = (uint32_t)(incl - c);
+      const int pre = incl - c;
+      S.bpref[lane] = (uint32_t)pre;
       S.bstart[lane] = bst + (uint32_t)(lane == ce ? co : 0);
+      // entry of chunk position k: the last non-empty entry starting at or
+      // before k (heads by position, then a running maximum)
+      S.head[lane] = 0xff;
       wave_sync();
+      if (c > 0 && pre < 64) S.head[pre] = (uint8_t)lane;
+      wave_sync();
+      const int hk = S.head[lane];
+      const int ek = wave_max_scan(hk == 0xff ? -1 : hk);
       const int k = lane;
       bool valid = false;
-      int ek = 0;
       uint32_t jidx = 0;
       double px = 0, py = 0, pz = 0, d2 = 0;
       if (k < total) {
-        int lo = 0, hi = lim - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if ((int)S.bpref[mid] <= k)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        ek = lo;
-        const uint32_t s = S.bstart[lo] + (uint32_t)(k - (int)S.bpref[lo]);
+        const uint32_t s = S.bstart[ek] + (uint32_t)(k - (int)S.bpref[ek]);
         jidx = A.sorted[s];
         valid = (jidx != self) && (A.target_group < 0 || A.groups[jidx] == A.target_group);
         px = A.spx[s] - sx;
@@ -1019,15 +1030,11 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         ce = 64;
         co = 0;
       } else {
-        int lo = 0, hi = lim - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if ((int)S.bpref[mid] <= 64)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        co = 64 - (int)S.bpref[lo] + (lo == ce ? co : 0);
+        // the entry holding position 64: the last non-empty one starting at
+        // or before it
+        const uint64_t hm = ballot(c > 0 && pre <= 64);
+        const int lo = 63 - __clzll((long long)hm);
+        co = 64 - rl_i(pre, lo) + (lo == ce ? co : 0);
         ce = lo;
       }
       wave_sync();
