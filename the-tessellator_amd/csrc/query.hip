@@ -38,6 +38,26 @@ __device__ __forceinline__ int cell_gap(int a, int b) {
   return d > 0 ? d : 0;
 }
 
+// Within one wavefront LDS operations execute in issue order: a phase
+// boundary only has to keep the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Running maximum over the lanes (inclusive), values >= -1: DPP row shifts
+// inside the rows of 16, then row broadcasts (lanes without a source: -1).
+__device__ __forceinline__ int wave_max_scan(int v) {
+  int t;
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:1
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:2
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:4
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:8
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false); v = t > v ? t : v;  // row_bcast:15
+  t = __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false); v = t > v ? t : v;  // row_bcast:31
+  return v;
+}
+
 // Exclusive prefix of v over the wavefront; *total = the sum.  DPP row
 // shifts inside each row of 16 lanes, then the row totals broadcast by
 // row_bcast:15 / :31 (lanes without a source add 0).
@@ -64,7 +84,8 @@ __device__ __forceinline__ uint32_t wave_excl(uint32_t v, int lane, uint32_t* to
 // (interface.rs:358-361); kept points are written in walk order (ballot rank).
 template <int MODE, bool FILL>
 __device__ __forceinline__ void emit_points(const QueryArgs& a, uint32_t s0, uint32_t len, int lane, double qx,
-                                            double qy, double qz, double r2, uint64_t& cursor, uint64_t end) {
+                                            double qy, double qz, double r2, uint64_t& cursor, uint64_t end,
+                                            uint8_t* head) {
   uint32_t total;
   const uint32_t start = wave_excl(len, lane, &total);
   if (MODE == 0 && !FILL) {
@@ -72,14 +93,21 @@ __device__ __forceinline__ void emit_points(const QueryArgs& a, uint32_t s0, uin
     return;
   }
   const uint64_t below = (1ull << lane) - 1ull;
+  int carry = 0;  // the cell holding the last position of the previous window
   for (uint32_t base = 0; base < total; base += 64) {
     const uint32_t p = base + (uint32_t)lane;
-    int c = 0;  // largest cell whose start is <= p (empty cells share the next start)
-#pragma unroll
-    for (int step = 32; step; step >>= 1) {
-      const uint32_t v = __shfl(start, c + step, 64);
-      if (v <= p) c += step;
-    }
+    // the cell of position p: the last non-empty cell starting at or before
+    // p (each writes its lane at its start in this window, then a running
+    // maximum; before the window's first start, the carried cell)
+    head[lane] = 0xff;
+    wave_sync();
+    if (len > 0 && start >= base && start - base < 64u) head[start - base] = (uint8_t)lane;
+    wave_sync();
+    const int h = head[lane];
+    int c = wave_max_scan(h == 0xff ? -1 : h);
+    c = c > carry ? c : carry;
+    carry = __builtin_amdgcn_readlane(c, 63);
+    wave_sync();
     const uint32_t cs0 = __shfl(s0, c, 64), cstart = __shfl(start, c, 64);
     const bool valid = p < total;
     const uint32_t s = cs0 + (p - cstart);
@@ -114,6 +142,8 @@ __device__ __forceinline__ void emit_points(const QueryArgs& a, uint32_t s0, uin
 // (KIND 1 cell radius, KIND 2 real radius): celery.rs:753-855.
 template <int KIND, bool FILL>
 __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
+  __shared__ uint8_t heads[4][64];
+  uint8_t* head = heads[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= a.nq) return;
@@ -165,7 +195,7 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
         s0 = a.delim[b];
         len = a.delim[b + 1] - s0;
       }
-      emit_points<KIND == 1 ? 0 : 1, FILL>(a, s0, len, lane, qx, qy, qz, r2, cursor, end);
+      emit_points<KIND == 1 ? 0 : 1, FILL>(a, s0, len, lane, qx, qy, qz, r2, cursor, end, head);
     }
   }
   if (lane == 0 && a.count) a.count[qi] = (uint32_t)(FILL ? cursor - a.offsets[qi] : cursor);
@@ -176,6 +206,8 @@ __global__ __launch_bounds__(256) void k_query_box(QueryArgs a) {
 // table covers every key <= the largest radius (or is the full order).
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_query_cloud(QueryArgs a) {
+  __shared__ uint8_t heads[4][64];
+  uint8_t* head = heads[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= a.nq) return;
@@ -207,9 +239,9 @@ __global__ __launch_bounds__(256) void k_query_cloud(QueryArgs a) {
       }
     }
     if (a.groups)
-      emit_points<2, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end);
+      emit_points<2, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end, head);
     else
-      emit_points<0, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end);
+      emit_points<0, FILL>(a, s0, len, lane, qx, qy, qz, 0.0, cursor, end, head);
     if (__ballot(beyond)) break;
   }
   if (lane == 0 && a.count) a.count[qi] = (uint32_t)(FILL ? cursor - a.offsets[qi] : cursor);
