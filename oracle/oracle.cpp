@@ -1170,8 +1170,14 @@ size_t orc_expand_all(const double* x, const double* y, const double* z, size_t 
 void orc_queries(int kind, const double* x, const double* y, const double* z, size_t n, double density,
                  const int64_t* groups, int64_t target_group, const double* q, size_t nq, const double* radius,
                  uint64_t* offsets, uint32_t* out) {
+  orc_queries_ordered(kind, x, y, z, n, density, groups, target_group, q, nq, radius, offsets, out, 0);
+}
+
+void orc_queries_ordered(int kind, const double* x, const double* y, const double* z, size_t n, double density,
+                         const int64_t* groups, int64_t target_group, const double* q, size_t nq,
+                         const double* radius, uint64_t* offsets, uint32_t* out, int order) {
   Grid G;
-  grid_setup(G, x, y, z, n, density, kind == 3);
+  grid_setup(G, x, y, z, n, density, kind == 3, order);
   const int64_t cpd = G.g.cpd;
   uint64_t c = 0;
   std::vector<uint32_t> tmp;

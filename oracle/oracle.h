@@ -101,6 +101,11 @@ size_t orc_find_neighbors(const double* px, const double* py, const double* pz, 
 void orc_queries(int kind, const double* x, const double* y, const double* z, size_t n, double density,
                  const int64_t* groups, int64_t target_group, const double* q, size_t nq, const double* radius,
                  uint64_t* offsets, uint32_t* out);
+/* orc_queries over a grid whose cells list their points in the given order
+ * (0: input index, 1: internal Morton order, see orc_grid_build_ordered). */
+void orc_queries_ordered(int kind, const double* x, const double* y, const double* z, size_t n, double density,
+                         const int64_t* groups, int64_t target_group, const double* q, size_t nq,
+                         const double* radius, uint64_t* offsets, uint32_t* out, int order);
 size_t orc_expand_all(const double* x, const double* y, const double* z, size_t n, double density,
                       double sxp, double syp, double szp, double max_radius, uint32_t* out);
 

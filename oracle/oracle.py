@@ -70,6 +70,8 @@ def lib():
             "orc_find_cells_in_radius": (sz, [C.POINTER(Grid)] + [C.c_double] * 4 + [dp]),
             "orc_find_neighbors": (sz, [dp, dp, dp, C.POINTER(Grid), dp, dp] + [C.c_double] * 4 + [C.c_int, dp]),
             "orc_queries": (None, [C.c_int, dp, dp, dp, sz, C.c_double, dp, C.c_int64, dp, sz, dp, dp, dp]),
+            "orc_queries_ordered": (None, [C.c_int, dp, dp, dp, sz, C.c_double, dp, C.c_int64, dp, sz, dp, dp, dp,
+                                           C.c_int]),
             "orc_morton_codes": (None, [dp, dp, dp, sz, dp]),
             "orc_index_maps": (None, [dp, dp, dp, sz, dp, dp]),
             "orc_grid_build_ordered": (None, [dp, dp, dp, sz, C.POINTER(Grid), C.c_int, dp, dp, dp]),
@@ -249,7 +251,7 @@ def find_neighbors(xyz, g, srt, dl, x, y, z, radius, real=False):
     return out
 
 
-def queries(kind, xyz, q, radius, density=1.25, groups=None, target_group=-1):
+def queries(kind, xyz, q, radius, density=1.25, groups=None, target_group=-1, order=0):
     """Many queries of one kind over the grid of xyz: (offsets (nq+1), ids)."""
     px, py, pz = soa(xyz)
     q = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1, 3))
@@ -258,9 +260,9 @@ def queries(kind, xyz, q, radius, density=1.25, groups=None, target_group=-1):
     g = None if groups is None else np.ascontiguousarray(groups, np.int64)
     off = np.empty(nq + 1, np.uint64)
     args = [kind, _p(px), _p(py), _p(pz), len(px), density, _p(g), int(target_group), _p(q), nq, _p(r), _p(off)]
-    lib().orc_queries(*args, None)
+    lib().orc_queries_ordered(*args, None, order)
     ids = np.empty(int(off[-1]), np.uint32)
-    lib().orc_queries(*args, _p(ids))
+    lib().orc_queries_ordered(*args, _p(ids), order)
     return off, ids
 
 

@@ -62,3 +62,17 @@ def test_morton_order_options(ctx):
     cells = d.find_cells_in_radius(0.5, 0.5, 0.5, 0.1)
     want = np.concatenate([srt[dl[c]:dl[c + 1]] for c in cells])
     assert np.array_equal(ids, want)  # a cell's points in the internal order
+
+
+@pytest.mark.parametrize("kind", [tess.QUERY_CELL_RADIUS, tess.QUERY_REAL_RADIUS, tess.QUERY_CLOUD])
+def test_morton_order_queries_match_oracle(ctx, kind):
+    """In the internal order a cell's points come by ascending internal index
+    in every query kind that lists points."""
+    pts = tess.clustered(20000, 97)
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0, point_order=tess.ORDER_MORTON)
+    rng = np.random.default_rng(98)
+    q = pts[rng.integers(0, len(pts), 5000)]
+    r = 0.02 if kind != tess.QUERY_CLOUD else 4e-4
+    off, ids = d.query(kind, q, r)
+    ro, rid = orc.queries(kind, pts, q, r, density=5.0, order=orc.ORDER_MORTON)
+    assert np.array_equal(off, ro) and np.array_equal(ids, rid)
