@@ -70,3 +70,21 @@ def _xoshiro_first(seed, k):
 def test_uniform_generator_is_xoshiro256ss():
     pts = tess.uniform(5, 1234)
     assert list(pts.reshape(-1)) == _xoshiro_first(1234, 15)
+
+
+def test_header_is_c99_and_struct_sizes_match(tmp_path):
+    """include/tess.h compiles as plain C99, and the binding's structures have
+    the sizes the C compiler gives them."""
+    import subprocess
+
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "tess.h"\nint main(void) {\n'
+                   '  printf("%zu %zu %zu %zu\\n", sizeof(tess_opts), sizeof(tess_cell_stats), '
+                   'sizeof(tess_timings), sizeof(tess_batch_config));\n  return 0;\n}\n')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                    os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == [ctypes.sizeof(tess.Opts), ctypes.sizeof(tess.CellStats), ctypes.sizeof(tess.Timings),
+                   ctypes.sizeof(tess.BatchConfig)]
