@@ -24,7 +24,10 @@ Workloads (--workload):
   geometry1m   vertices and face rings of every cell of the 1,000,000 uniform
                points (seed 1): the geometry call after a compute (a cell
                pass that stages the geometry, offsets, compaction), results
-               left in HBM; reports cells/s of the export.
+               left in HBM; reports cells/s of the export, and beside it
+               cells + geometry by one call on a diagram not yet computed
+               (its compute pass stages the geometry) against compute then
+               geometry.
   queries1m    radius queries (find_neighbors_in_real_radius, r = 0.01) at
                the 1,000,000 points of the clustered set (seed 25, grid of
                about 5 points per cell), positions and radii resident in
@@ -284,6 +287,27 @@ def bench_geometry(ctx, w, args):
     ms = float(np.mean(ms_list))
     nv, nr = C.c_uint64(), C.c_uint64()
     d.ctx.check(lib.tess_diagram_geometry_sizes(d.h, C.byref(nv), C.byref(nr)))
+    # cells and geometry wanted together: compute then geometry (two cell
+    # passes) against one geometry call on a diagram not yet computed, whose
+    # compute pass stages the geometry itself (fresh diagrams built untimed)
+    sep, one = [], []
+    for i in range(1 + min(args.steps, 3)):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        d.compute()
+        step()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        f = tess.Diagram(ctx, pts, np.zeros(3), np.ones(3), density=w["density"])
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        f.ctx.check(lib.tess_diagram_compute_geometry(f.h))
+        ctx.synchronize()
+        t3 = time.perf_counter()
+        f.close()
+        if i:
+            sep.append((t1 - t0) * 1e3)
+            one.append((t3 - t2) * 1e3)
     stage_ms = mean(tims, "stage_ms")
     achieved = flops / (stage_ms / 1e3) / 1e12
     line = {
@@ -301,11 +325,13 @@ def bench_geometry(ctx, w, args):
         "data": "synthetic (seeded xoshiro256**)",
         "config": {"workload": w["desc"], "points": n, "cell_density": w["density"]},
         "phases_ms": {"stage": stage_ms, "compact": mean(tims, "compact_ms")},
+        "cells_and_geometry_ms": {"compute_then_geometry": float(np.mean(sep)),
+                                  "one_call_staged_by_compute": float(np.mean(one))},
         "sizes": {"vertices": nv.value, "ring_entries": nr.value, "faces": int(d.cells()["face_count"].sum())},
         "roofline": {
             "bound": "mfma",
             "ceiling": "FP64 78.6 TFLOP/s (the staging pass recomputes every cell)",
-            "kernel": "staging pass k_cells<1, 2, 1, true> (+ its capacity rerun), HIP events on the context "
+            "kernel": "staging pass k_cells<1, 2, 1, 1> (+ its capacity rerun), HIP events on the context "
                       "stream",
             "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops, "kernel_ms": stage_ms,

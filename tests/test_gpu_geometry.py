@@ -51,6 +51,31 @@ def test_rings_match_oracle(ctx, workload):
     assert np.all(nv - ne + cells["face_count"] == 2)
 
 
+def test_geometry_staged_by_compute(ctx):
+    """Geometry of a diagram not yet computed: the compute's own cell passes
+    (first pass and capacity rerun) stage it. Cell outputs and counters equal
+    a plain compute's; rings equal the oracle's."""
+    pts = tess.clustered(20000, 86)
+    plain = tess.Diagram(ctx, pts, LO, HI, density=5.0, collect_stats=True)
+    plain.compute()
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0, collect_stats=True)
+    geo = d.geometry()
+    cells = d.cells()
+    ref_cells = plain.cells()
+    for k in ref_cells:
+        assert np.array_equal(ref_cells[k], cells[k]), k
+    assert np.array_equal(plain.stats(), d.stats())
+    assert d.timings()["capacity_reruns"] > 0
+    assert geo["timings"]["stage_ms"] == d.timings()["cells_ms"]
+    ref = orc.compute_geometry(pts, LO, HI, density=5.0)
+    assert np.array_equal(geo["ring_offset"], ref["ring_offset"])
+    xyz, _ = _ring_coords(d, cells, geo)
+    assert np.array_equal(xyz, ref["ring_xyz"])
+    again = d.geometry()  # computed now: the separate staging pass
+    for k in ("vertex_offset", "vertices", "ring_offset", "ring"):
+        assert np.array_equal(geo[k], again[k]), k
+
+
 def test_ring_areas_and_orientation(ctx):
     """The fan area of every ring equals the face area; rings are
     counter-clockwise seen from outside (the site is behind every face)."""

@@ -282,6 +282,15 @@ tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, 
   a.xcd_chunk = work ? 0 : plain_xcd_chunk(d);
   a.redo = push_redo ? d->redo.p + 1 : nullptr;
   a.nredo = push_redo ? d->redo.p : nullptr;
+  if (GeoStage* g = d->geo_stage) {  // the same pass, staging the geometry too
+    a.geo_count = g->count;
+    GeoStaging& b = work ? g->rerun : g->first;
+    a.geo_slot = b.slot.p;
+    a.geo_v = b.v.p;
+    a.geo_ring = b.ring.p;
+    a.geo_roff = b.roff.p;
+    return c->hip(launch_cells_geometry(vpl, a, true, c->stream), "cell kernel");
+  }
   return c->hip(launch_cells(vpl, a, c->stream), "cell kernel");
 }
 
@@ -351,6 +360,10 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
     return st;
   if (nredo) {
     if (d->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if (GeoStage* g = d->geo_stage) {  // staging of the rerun items
+      g->nredo = nredo;
+      if ((st = geo_staging_alloc(d, g->rerun, nredo, 4)) != TESS_OK) return st;
+    }
     if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
                      "redo")) != TESS_OK ||
         (st = c->hip(hipMemsetAsync(&d->summary.p->capacity, 0, sizeof(unsigned int), s), "memset")) != TESS_OK ||

@@ -728,7 +728,8 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
 
 // One cell: the site at grid position gpos of the set A.  Returns the
 // cell's status bits (wave-uniform); the caller queues capacity reruns.
-template <int VPL, int PPL, int FPL, bool GEO = false>
+// GEO: 0 the cell outputs, 1 the staged geometry only, 2 both.
+template <int VPL, int PPL, int FPL, int GEO = 0>
 __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos, Lds<VPL, PPL, FPL>& S,
                                               const int lane, const int slot = 0) {
   using L = Lds<VPL, PPL, FPL>;
@@ -1148,13 +1149,13 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       if (fal[r]) S.u.fcontrib[rs[r]] = contrib[r];
     wave_sync();
     // (the geometry passes write only the geometry)
-    if (!GEO && lane == 0) {
+    if (GEO != 1 && lane == 0) {
       double vol = 0.0;
       for (int f = 0; f < nfaces; ++f) vol = vol + S.u.fcontrib[f];
       A.volume[oi] = vol / 6.0;
       A.fcount[oi] = (uint32_t)nfaces;
     }
-    if (!GEO) {
+    if (GEO != 1) {
 #pragma unroll
       for (int r = 0; r < FPL; ++r)
         if (fal[r]) {
@@ -1189,7 +1190,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         A.exp_meta[1] = nfaces;
       }
     }
-    if constexpr (GEO) {
+    if constexpr (GEO != 0) {
       // Geometry of whole sets (compute_vertices / compute_face_vertices,
       // polyhedron.rs:885-919), staged at this work item's slot: the live
       // vertices in slot order, and per face in output order the ring of
@@ -1251,7 +1252,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   }
   oflops = wave_sum_u64(oflops);
   if (lane == 0) {
-    if (!GEO) {
+    if (GEO != 1) {
       if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) {
         A.volume[oi] = 0.0;
         A.fcount[oi] = 0;
@@ -1267,7 +1268,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       if (status & TESS_CELL_CAPACITY) atomicAdd(&A.summary->capacity, 1u);
       if (status & TESS_CELL_TABLE) atomicAdd(&A.summary->table, 1u);
     }
-    if (!GEO && A.stats) {
+    if (GEO != 1 && A.stats) {
       tess_cell_stats st;
       st.streamed = streamed;
       st.classified = classified;
@@ -1285,7 +1286,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 }
 
 // One cell per wavefront over a set's grid positions (all, or a work list).
-template <int VPL, int PPL, int FPL, bool GEO = false>
+template <int VPL, int PPL, int FPL, int GEO = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells(CellArgs A) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
@@ -1421,13 +1422,17 @@ hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s) {
 static_assert(Lds<1, 2, 1>::HC == kCellHalfEdges(1) && Lds<4, 6, 2>::HC == kCellHalfEdges(4),
               "geometry staging capacity");
 
-hipError_t launch_cells_geometry(int vpl, const CellArgs& a, hipStream_t s) {
+hipError_t launch_cells_geometry(int vpl, const CellArgs& a, bool with_cells, hipStream_t s) {
   if (a.nwork <= 0) return hipSuccess;
   const int nb = a.xcd_chunk ? (int)xcd_grid((uint32_t)a.nwork, (uint32_t)a.xcd_chunk) : a.nwork;
-  if (vpl == 1)
-    hipLaunchKernelGGL((k_cells<1, 2, 1, true>), dim3(nb), dim3(64), 0, s, a);
+  if (vpl == 1 && !with_cells)
+    hipLaunchKernelGGL((k_cells<1, 2, 1, 1>), dim3(nb), dim3(64), 0, s, a);
+  else if (vpl == 4 && !with_cells)
+    hipLaunchKernelGGL((k_cells<4, 6, 2, 1>), dim3(nb), dim3(64), 0, s, a);
+  else if (vpl == 1)
+    hipLaunchKernelGGL((k_cells<1, 2, 1, 2>), dim3(nb), dim3(64), 0, s, a);
   else if (vpl == 4)
-    hipLaunchKernelGGL((k_cells<4, 6, 2, true>), dim3(nb), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((k_cells<4, 6, 2, 2>), dim3(nb), dim3(64), 0, s, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

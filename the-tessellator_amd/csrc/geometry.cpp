@@ -6,86 +6,41 @@
 #include "launch.h"
 #include "runtime.h"
 
-extern "C" {
-
-tess_status tess_diagram_compute_geometry(tess_diagram* d) {
-  if (!d) return TESS_E_INVALID;
+tess_status geo_staging_alloc(tess_diagram* d, GeoStaging& g, size_t items, int vpl) {
   tess_ctx* c = d->ctx;
-  std::lock_guard<std::mutex> lk(c->mu);
-  (void)hipSetDevice(c->device);
-  hipStream_t s = c->stream;
-  tess_status st;
-  if (!d->computed && (st = compute_cells(d, nullptr)) != TESS_OK) return st;
-  if (d->degenerate) return c->fail(TESS_E_DEGENERATE, "the diagram has failed cells");
-  const size_t n = d->n;
-  const GridDev& G = d->hgrid;
-  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
-  if (!t) return st;
-  // One pass recomputes every cell like compute does (capacity 64, the
-  // overflows again at 256) and stages its geometry per work item; the
-  // counts give the CSR offsets and a compaction copies the staged data.
-  const size_t v1 = 64, h1 = kCellHalfEdges(1), v4 = 256, h4 = kCellHalfEdges(4);
-  DevBuf<uint32_t> slot1, slot4;
-  DevBuf<double> sv1, sv4;
-  DevBuf<uint16_t> sr1, sr4, sro1, sro4;
-  if (d->geo_count.alloc(2 * n) != hipSuccess || d->geo_voff.alloc(n + 1) != hipSuccess ||
-      d->geo_rbase.alloc(n + 1) != hipSuccess || d->geo_sums.alloc((n + 1023) / 1024 + 1) != hipSuccess ||
-      slot1.alloc(n) != hipSuccess || sv1.alloc(3 * v1 * n) != hipSuccess || sr1.alloc(h1 * n) != hipSuccess ||
-      sro1.alloc((size_t)kFaceStage * n) != hipSuccess)
+  g.vpl = vpl;
+  if (g.slot.alloc(items) != hipSuccess || g.v.alloc(3ull * 64 * vpl * items) != hipSuccess ||
+      g.ring.alloc((size_t)kCellHalfEdges(vpl) * items) != hipSuccess ||
+      g.roff.alloc((size_t)kFaceStage * items) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  CellArgs a = base_args(d, t);
-  a.work = nullptr;
-  a.nwork = (int)n;
-  a.xcd_chunk = plain_xcd_chunk(d);
-  a.stats = nullptr;
-  a.summary = nullptr;
-  a.redo = d->redo.p + 1;
-  a.nredo = d->redo.p;
-  a.geo_count = d->geo_count.p;
-  a.geo_slot = slot1.p;
-  a.geo_v = sv1.p;
-  a.geo_ring = sr1.p;
-  a.geo_roff = sro1.p;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  for (auto& e : ev)
-    if (hipEventCreate(&e) != hipSuccess) return c->fail(TESS_E_DEVICE, "event create failed");
-  struct Release {
-    hipEvent_t* e;
-    ~Release() {
-      for (int k = 0; k < 3; ++k) (void)hipEventDestroy(e[k]);
-    }
-  } release{ev};
-  uint32_t nredo = 0;
-  if ((st = c->hip(launch_fill_u32(d->geo_count.p, 2 * n, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32(slot1.p, n, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(hipEventRecord(ev[0], s), "event")) != TESS_OK ||
-      (st = c->hip(launch_cells_geometry(1, a, s), "geometry")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
-      (st = c->hip(hipStreamSynchronize(s), "geometry")) != TESS_OK)
-    return st;
-  if (nredo) {
-    if (d->redo_list.alloc(nredo) != hipSuccess || slot4.alloc(nredo) != hipSuccess ||
-        sv4.alloc(3 * v4 * nredo) != hipSuccess || sr4.alloc(h4 * nredo) != hipSuccess ||
-        sro4.alloc((size_t)kFaceStage * nredo) != hipSuccess)
-      return c->fail(TESS_E_NOMEM, "device allocation failed");
-    CellArgs r = a;
-    r.work = d->redo_list.p;
-    r.nwork = (int)nredo;
-    r.xcd_chunk = 0;
-    r.redo = nullptr;
-    r.nredo = nullptr;
-    r.geo_slot = slot4.p;
-    r.geo_v = sv4.p;
-    r.geo_ring = sr4.p;
-    r.geo_roff = sro4.p;
-    if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
-                     "redo")) != TESS_OK ||
-        (st = c->hip(launch_fill_u32(slot4.p, nredo, 0u, s), "fill")) != TESS_OK ||
-        (st = c->hip(launch_cells_geometry(4, r, s), "geometry")) != TESS_OK)
-      return st;
+  return c->hip(launch_fill_u32(g.slot.p, items, 0u, c->stream), "fill");
+}
+
+namespace {
+
+struct GeoEvents {
+  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+  bool ok = true;
+  GeoEvents() {
+    for (auto& x : e)
+      if (hipEventCreate(&x) != hipSuccess) ok = false;
   }
-  if ((st = c->hip(hipEventRecord(ev[1], s), "event")) != TESS_OK ||
+  ~GeoEvents() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+// Per-cell counts (d->geo_count, staged with the cells) to CSR offsets, then
+// the staged vertices and rings of both passes compacted into the CSR arrays.
+// ev[1] is recorded here; ev[0], when recorded, opens the staging time.
+tess_status finish_geometry(tess_diagram* d, const GeoStaging& first, const GeoStaging& rerun, uint32_t nredo,
+                            GeoEvents& ev, double stage_ms) {
+  tess_ctx* c = d->ctx;
+  hipStream_t s = c->stream;
+  const size_t n = d->n;
+  tess_status st;
+  if ((st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
       (st = c->hip(launch_face_offsets(d->geo_count.p, (int)n, d->geo_voff.p, d->geo_sums.p, s), "scan")) !=
           TESS_OK ||
       (st = c->hip(launch_face_offsets(d->geo_count.p + n, (int)n, d->geo_rbase.p, d->geo_sums.p, s), "scan")) !=
@@ -100,26 +55,108 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   if (d->geo_v.alloc(3 * d->geo_nv) != hipSuccess || d->geo_roff.alloc(F + 1) != hipSuccess ||
       d->geo_ring.alloc(d->geo_nr) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  float ms[2] = {0, 0};
-  if ((st = c->hip(launch_geo_compact(slot1.p, (int)n, (int)v1, (int)h1, sv1.p, sr1.p, sro1.p, d->geo_count.p,
-                                      (int)n, d->geo_voff.p, d->geo_rbase.p, d->foff.p, d->geo_v.p, d->geo_ring.p,
-                                      d->geo_roff.p, s),
-                   "compact")) != TESS_OK ||
-      (nredo && (st = c->hip(launch_geo_compact(slot4.p, (int)nredo, (int)v4, (int)h4, sv4.p, sr4.p, sro4.p,
-                                                d->geo_count.p, (int)n, d->geo_voff.p, d->geo_rbase.p, d->foff.p,
-                                                d->geo_v.p, d->geo_ring.p, d->geo_roff.p, s),
-                             "compact")) != TESS_OK) ||
+  auto compact = [&](const GeoStaging& g, size_t items) {
+    return c->hip(launch_geo_compact(g.slot.p, (int)items, 64 * g.vpl, kCellHalfEdges(g.vpl), g.v.p, g.ring.p,
+                                     g.roff.p, d->geo_count.p, (int)n, d->geo_voff.p, d->geo_rbase.p, d->foff.p,
+                                     d->geo_v.p, d->geo_ring.p, d->geo_roff.p, s),
+                  "compact");
+  };
+  float ms[2] = {(float)stage_ms, 0.f};
+  if ((st = compact(first, n)) != TESS_OK || (nredo && (st = compact(rerun, nredo)) != TESS_OK) ||
       (st = c->hip(hipMemcpyAsync(d->geo_roff.p + F, &d->geo_nr, 8, hipMemcpyHostToDevice, s), "upload")) !=
           TESS_OK ||
-      (st = c->hip(hipEventRecord(ev[2], s), "event")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev.e[2], s), "event")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "geometry")) != TESS_OK ||
-      (st = c->hip(hipEventElapsedTime(&ms[0], ev[0], ev[1]), "event")) != TESS_OK ||
-      (st = c->hip(hipEventElapsedTime(&ms[1], ev[1], ev[2]), "event")) != TESS_OK)
+      (stage_ms < 0 && (st = c->hip(hipEventElapsedTime(&ms[0], ev.e[0], ev.e[1]), "event")) != TESS_OK) ||
+      (st = c->hip(hipEventElapsedTime(&ms[1], ev.e[1], ev.e[2]), "event")) != TESS_OK)
     return st;
   d->geo_ms[0] = ms[0];
   d->geo_ms[1] = ms[1];
   d->geo_ready = true;
   return TESS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tess_status tess_diagram_compute_geometry(tess_diagram* d) {
+  if (!d) return TESS_E_INVALID;
+  tess_ctx* c = d->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  tess_status st;
+  const size_t n = d->n;
+  GeoEvents ev;
+  if (!ev.ok) return c->fail(TESS_E_DEVICE, "event create failed");
+  if (d->geo_count.alloc(2 * n) != hipSuccess || d->geo_voff.alloc(n + 1) != hipSuccess ||
+      d->geo_rbase.alloc(n + 1) != hipSuccess || d->geo_sums.alloc((n + 1023) / 1024 + 1) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  GeoStage g;
+  g.count = d->geo_count.p;
+  if (!d->computed && pick_vpl(d) == 1) {
+    // not computed yet: the compute's own cell passes stage the geometry
+    // (capacity 64, the overflows again at 256), then only the offsets and
+    // the compaction remain
+    if ((st = geo_staging_alloc(d, g.first, n, 1)) != TESS_OK ||
+        (st = c->hip(launch_fill_u32(d->geo_count.p, 2 * n, 0u, s), "fill")) != TESS_OK)
+      return st;
+    d->geo_stage = &g;
+    st = compute_cells(d, nullptr);
+    d->geo_stage = nullptr;
+    if (st != TESS_OK) return st;
+    return finish_geometry(d, g.first, g.rerun, g.nredo, ev, d->tim.cells_ms);
+  }
+  if (!d->computed && (st = compute_cells(d, nullptr)) != TESS_OK) return st;
+  if (d->degenerate) return c->fail(TESS_E_DEGENERATE, "the diagram has failed cells");
+  const GridDev& G = d->hgrid;
+  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
+  if (!t) return st;
+  // One pass recomputes every cell like compute does (capacity 64, the
+  // overflows again at 256) and stages its geometry per work item; the
+  // counts give the CSR offsets and a compaction copies the staged data.
+  if ((st = geo_staging_alloc(d, g.first, n, 1)) != TESS_OK) return st;
+  CellArgs a = base_args(d, t);
+  a.work = nullptr;
+  a.nwork = (int)n;
+  a.xcd_chunk = plain_xcd_chunk(d);
+  a.stats = nullptr;
+  a.summary = nullptr;
+  a.redo = d->redo.p + 1;
+  a.nredo = d->redo.p;
+  a.geo_count = d->geo_count.p;
+  a.geo_slot = g.first.slot.p;
+  a.geo_v = g.first.v.p;
+  a.geo_ring = g.first.ring.p;
+  a.geo_roff = g.first.roff.p;
+  uint32_t nredo = 0;
+  if ((st = c->hip(launch_fill_u32(d->geo_count.p, 2 * n, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
+      (st = c->hip(launch_cells_geometry(1, a, false, s), "geometry")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "geometry")) != TESS_OK)
+    return st;
+  if (nredo) {
+    if (d->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if ((st = geo_staging_alloc(d, g.rerun, nredo, 4)) != TESS_OK) return st;
+    CellArgs r = a;
+    r.work = d->redo_list.p;
+    r.nwork = (int)nredo;
+    r.xcd_chunk = 0;
+    r.redo = nullptr;
+    r.nredo = nullptr;
+    r.geo_slot = g.rerun.slot.p;
+    r.geo_v = g.rerun.v.p;
+    r.geo_ring = g.rerun.ring.p;
+    r.geo_roff = g.rerun.roff.p;
+    if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
+                     "redo")) != TESS_OK ||
+        (st = c->hip(launch_cells_geometry(4, r, false, s), "geometry")) != TESS_OK)
+      return st;
+  }
+  return finish_geometry(d, g.first, g.rerun, nredo, ev, -1.0);
 }
 
 tess_status tess_diagram_geometry_sizes(const tess_diagram* d, uint64_t* nvertices, uint64_t* nring) {

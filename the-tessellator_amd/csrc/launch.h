@@ -42,10 +42,10 @@ hipError_t launch_invert(const uint32_t* sorted, int n, uint32_t* inv, hipStream
 
 // cell.hip: vpl = vertex slots per lane (1, 2 or 4)
 hipError_t launch_cells(int vpl, const CellArgs& a, hipStream_t s);
-// the same cells with the geometry export only (CellArgs::geo_*; the other
-// outputs are not written), vpl 1 or 4; staging capacity per work item:
+// the same cells staging their geometry (CellArgs::geo_*), with the cell
+// outputs too when with_cells, vpl 1 or 4; staging capacity per work item:
 // 64 vpl vertices, kCellHalfEdges(vpl) ring entries, kFaceStage faces
-hipError_t launch_cells_geometry(int vpl, const CellArgs& a, hipStream_t s);
+hipError_t launch_cells_geometry(int vpl, const CellArgs& a, bool with_cells, hipStream_t s);
 constexpr int kCellHalfEdges(int vpl) { return vpl == 1 ? 256 : 768; }
 // geometry.hip: staged geometry of nslots work items -> the CSR
 hipError_t launch_geo_compact(const uint32_t* slot_cell, int nslots, int vcap, int hcap, const double* sv,

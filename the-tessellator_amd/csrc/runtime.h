@@ -124,6 +124,27 @@ struct tess_cell {
   std::vector<double> verts;
 };
 
+// Geometry of one cell pass staged per work item: slot (output index + 1),
+// vertices, ring entries and ring offsets at the pass's capacity.
+struct GeoStaging {
+  int vpl = 1;
+  DevBuf<uint32_t> slot;
+  DevBuf<double> v;
+  DevBuf<uint16_t> ring, roff;
+};
+
+// Staging filled by the cell passes of a compute itself (the geometry of a
+// diagram not yet computed): the first pass per grid position, the capacity
+// rerun per rerun item.
+struct GeoStage {
+  uint32_t* count = nullptr;  // [2n]: live vertices, ring entries per cell
+  GeoStaging first, rerun;
+  uint32_t nredo = 0;
+};
+
+// Allocates g for `items` work items at capacity 64 vpl and clears its slots.
+tess_status geo_staging_alloc(tess_diagram* d, GeoStaging& g, size_t items, int vpl);
+
 struct tess_diagram {
   tess_ctx* ctx = nullptr;
   tess_opts opts;
@@ -170,6 +191,7 @@ struct tess_diagram {
   DevBuf<uint16_t> geo_ring;
   uint64_t geo_nv = 0, geo_nr = 0;
   double geo_ms[2] = {0, 0};  // staging pass (+ rerun), scans + compaction
+  GeoStage* geo_stage = nullptr;  // set while a compute stages the geometry
   bool geo_ready = false;
   uint64_t total_faces = 0;
   tess_timings tim;
