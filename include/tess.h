@@ -205,8 +205,10 @@ size_t tess_cell_vertices(const tess_cell* c, double* xyz, size_t cap);
 
 /* ---- geometry of every cell (interface.rs:367-370, :403-406) -----------
  * Cell::compute_vertices and VoronoiFace::compute_vertices
- * (polyhedron.rs:885-919) for all cells at once, as CSR.  Runs compute
- * first if needed.  Vertices are site-relative, per cell in the kernel's
+ * (polyhedron.rs:885-919) for all cells at once, as CSR.  On a diagram not
+ * yet computed it runs the compute, whose cell passes stage the geometry as
+ * well (one cell pass for both, at the default vertex capacity); after a
+ * compute it runs one staging pass.  Vertices are site-relative, per cell in the kernel's
  * slot order (the reference's pool order is not reproducible);
  * vertex_offset[n + 1] indexes them.  Each face (in the order of the
  * neighbour lists, so face f of the whole diagram is neighbor[f]) has a ring
