@@ -76,6 +76,20 @@ def test_geometry_staged_by_compute(ctx):
         assert np.array_equal(geo[k], again[k]), k
 
 
+def test_geometry_at_capacity_256(ctx):
+    """A diagram whose compute runs at vertex capacity 256 takes the separate
+    staging pass; its rings equal those staged by a default compute."""
+    pts = tess.clustered(8000, 87)
+    ref = tess.Diagram(ctx, pts, LO, HI, density=5.0)
+    g_ref = ref.geometry()
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0, vertex_capacity=256)
+    geo = d.geometry()
+    cells = d.cells()
+    assert np.array_equal(cells["neighbor"], ref.cells()["neighbor"])
+    assert np.array_equal(geo["ring_offset"], g_ref["ring_offset"])
+    assert np.array_equal(_ring_coords(d, cells, geo)[0], _ring_coords(ref, ref.cells(), g_ref)[0])
+
+
 def test_ring_areas_and_orientation(ctx):
     """The fan area of every ring equals the face area; rings are
     counter-clockwise seen from outside (the site is behind every face)."""
