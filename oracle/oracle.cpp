@@ -1461,11 +1461,17 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
 orc_result* orc_compute_geometry(const double* x, const double* y, const double* z, size_t n, const double lo[3],
                                  const double hi[3], double density, const uint32_t* cells, size_t ncells,
                                  int nthreads) {
-  orc_result* R = orc_compute(x, y, z, n, lo, hi, density, ORC_MODE_SECURITY, ORC_CLIP_FACES, cells, ncells,
-                              nthreads);
+  return orc_compute_geometry_ordered(x, y, z, n, lo, hi, density, cells, ncells, nthreads, 0);
+}
+
+orc_result* orc_compute_geometry_ordered(const double* x, const double* y, const double* z, size_t n,
+                                         const double lo[3], const double hi[3], double density,
+                                         const uint32_t* cells, size_t ncells, int nthreads, int order) {
+  orc_result* R = orc_compute_ordered(x, y, z, n, lo, hi, density, ORC_MODE_SECURITY, ORC_CLIP_FACES, cells, ncells,
+                                      nthreads, std::numeric_limits<double>::quiet_NaN(), nullptr, -1, order);
   if (n == 0) return R;
   Grid G;
-  grid_setup(G, x, y, z, n, density, false);
+  grid_setup(G, x, y, z, n, density, false, order);
   const size_t nc = R->ncells;
   std::vector<std::vector<Ring>> per(nc);
   Filter flt;

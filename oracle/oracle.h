@@ -176,6 +176,11 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
 orc_result* orc_compute_geometry(const double* x, const double* y, const double* z, size_t n, const double lo[3],
                                  const double hi[3], double density, const uint32_t* cells, size_t ncells,
                                  int nthreads);
+/* orc_compute_geometry with the candidate order of a grid cell chosen by
+ * order (as orc_compute_ordered). */
+orc_result* orc_compute_geometry_ordered(const double* x, const double* y, const double* z, size_t n,
+                                         const double lo[3], const double hi[3], double density,
+                                         const uint32_t* cells, size_t ncells, int nthreads, int order);
 uint64_t orc_result_ring_entries(const orc_result* R);
 /* ring_offset: total faces + 1 entries (into the entries); xyz: 3 per entry */
 void orc_result_copy_rings(const orc_result* R, uint64_t* ring_offset, double* xyz);

@@ -93,6 +93,7 @@ def lib():
             "orc_compute_at_points": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int,
                                            C.c_double, dp, C.c_int64]),
             "orc_compute_geometry": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, dp, sz, C.c_int]),
+            "orc_compute_geometry_ordered": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, dp, sz, C.c_int, C.c_int]),
             "orc_result_ring_entries": (C.c_uint64, [vp]),
             "orc_result_copy_rings": (None, [vp, dp, dp]),
             "orc_result_cells": (sz, [vp]),
@@ -339,16 +340,17 @@ def compute_at_points(xyz, sites, lo, hi, density=1.25, mode=MODE_SECURITY, clip
     return _result(r, len(st))
 
 
-def compute_geometry(xyz, lo, hi, density=1.25, cells=None, threads=0):
+def compute_geometry(xyz, lo, hi, density=1.25, cells=None, threads=0, order=0):
     """compute() plus every face's vertex ring: ring_offset (faces + 1) into
     ring_xyz (entries x 3), faces in output order, rings from the
-    lexicographically smallest vertex, site-relative."""
+    lexicographically smallest vertex, site-relative. order as compute()."""
     x, y, z = soa(xyz)
     n = len(x)
     lo, hi = _v(lo), _v(hi)
     cl = None if cells is None else np.ascontiguousarray(cells, np.uint32)
     nc = n if cl is None else len(cl)
-    r = lib().orc_compute_geometry(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, _p(cl), nc, threads)
+    r = lib().orc_compute_geometry_ordered(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, _p(cl), nc, threads,
+                                           order)
     m = lib().orc_result_ring_entries(r)
     f = lib().orc_result_total_faces(r)
     ro, rx = np.empty(f + 1, np.uint64), np.empty((m, 3))

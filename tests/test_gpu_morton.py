@@ -76,3 +76,24 @@ def test_morton_order_queries_match_oracle(ctx, kind):
     off, ids = d.query(kind, q, r)
     ro, rid = orc.queries(kind, pts, q, r, density=5.0, order=orc.ORDER_MORTON)
     assert np.array_equal(off, ro) and np.array_equal(ids, rid)
+
+
+@pytest.mark.parametrize("computed", [False, True])
+def test_morton_order_geometry_matches_oracle(ctx, computed):
+    """Vertices and face rings of a diagram in the internal order, staged by
+    its compute pass or by the separate staging pass, bitwise against the
+    oracle's rings in that order."""
+    pts = tess.clustered(20000, 97)
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0, point_order=tess.ORDER_MORTON)
+    if computed:
+        d.compute()
+    geo = d.geometry()
+    cells = d.cells()
+    ref = orc.compute_geometry(pts, LO, HI, density=5.0, order=orc.ORDER_MORTON)
+    for k in ("volume", "neighbor", "area"):
+        assert np.array_equal(cells[k], ref[k]), k
+    assert np.array_equal(geo["ring_offset"], ref["ring_offset"])
+    fc = cells["face_count"].astype(np.int64)
+    entry_cell = np.repeat(np.repeat(np.arange(len(fc)), fc), np.diff(geo["ring_offset"].astype(np.int64)))
+    xyz = geo["vertices"][geo["vertex_offset"][entry_cell].astype(np.int64) + geo["ring"]]
+    assert np.array_equal(xyz, ref["ring_xyz"])

@@ -74,3 +74,24 @@ def test_cells_in_internal_order_are_the_same_cells():
     assert np.allclose(a["volume"], b["volume"], rtol=1e-12, atol=0)
     for k in ("volume", "neighbor", "area"):
         assert np.array_equal(b[k], c[k]), k
+
+
+def test_geometry_in_internal_order():
+    """Rings in the internal order: the cells of compute() in that order
+    (bitwise), the same rings as in input order up to rounding, and fan areas
+    equal to the face areas."""
+    pts = tess.uniform(2000, 35)
+    lo, hi = np.zeros(3), np.ones(3)
+    g = orc.compute_geometry(pts, lo, hi, density=5.0, order=orc.ORDER_MORTON)
+    c = orc.compute(pts, lo, hi, density=5.0, order=orc.ORDER_MORTON)
+    for k in ("volume", "neighbor", "area"):
+        assert np.array_equal(g[k], c[k]), k
+    g0 = orc.compute_geometry(pts, lo, hi, density=5.0)
+    assert np.array_equal(g["ring_offset"], g0["ring_offset"])
+    assert np.allclose(g["ring_xyz"], g0["ring_xyz"], rtol=0, atol=1e-12)
+    assert not np.array_equal(g["ring_xyz"], g0["ring_xyz"])  # the order did reach the clip
+    ro = g["ring_offset"].astype(np.int64)
+    for f in range(0, len(ro) - 1, 53):
+        ring = g["ring_xyz"][ro[f]:ro[f + 1]]
+        w = sum(np.cross(ring[k] - ring[0], ring[k + 1] - ring[0]) for k in range(1, len(ring) - 1))
+        assert abs(0.5 * np.linalg.norm(w) - g["area"][f]) <= 1e-12 * max(g["area"][f], 1e-12)
