@@ -28,6 +28,12 @@ Workloads (--workload):
                cells + geometry by one call on a diagram not yet computed
                (its compute pass stages the geometry) against compute then
                geometry.
+  sites1m      cells of 1,000,000 uniform sites that are not points of the
+               diagram of the 1,000,000 uniform points (get_cell_at_particle
+               for every site, tess_diagram_cells_at_particles); the sites
+               arrive as a host array, so cells/s is taken over the device
+               span of the call (cell passes, offsets, compaction) and the
+               whole call, upload included, is reported beside it.
   queries1m    radius queries (find_neighbors_in_real_radius, r = 0.01) at
                the 1,000,000 points of the clustered set (seed 25, grid of
                about 5 points per cell), positions and radii resident in
@@ -67,6 +73,10 @@ WORKLOADS = {
     "geometry1m": dict(desc="geometry export (vertices + face rings, CSR) of every cell, 1,000,000 points i.i.d. "
                             "uniform in [0,1]^3, f64, box walls", points=1_000_000, seed=1, density=5.0,
                        kind="geometry"),
+    "sites1m": dict(desc="cells of 1,000,000 sites that are not points of the diagram (get_cell_at_particle), "
+                         "sites i.i.d. uniform (seed 2) against 1,000,000 points i.i.d. uniform (seed 1) in "
+                         "[0,1]^3, f64, box walls", points=1_000_000, seed=1, sites=1_000_000, site_seed=2,
+                    density=5.0, kind="sites"),
     "queries1m": dict(desc="real-radius neighbour queries (r = 0.01) at the 1,000,000 points of the clustered set "
                            "(64 Gaussians sigma 0.02 + 20% uniform, seed 25), f64",
                       points=1_000_000, seed=25, density=5.0, kind="queries", radius=0.01),
@@ -342,6 +352,86 @@ def bench_geometry(ctx, w, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_sites(ctx, w, args):
+    """tess_diagram_cells_at_particles for 1M sites: one step is one call
+    (upload, first cell pass, capacity rerun, offsets, compaction)."""
+    import ctypes as C
+
+    pts = tess.uniform(w["points"], w["seed"])
+    sites = np.ascontiguousarray(tess.uniform(w["sites"], w["site_seed"]))
+    n, m = len(pts), len(sites)
+    lo, hi = np.zeros(3), np.ones(3)
+    # exact FP64 operation count of the cells (untimed, counters on)
+    ds = tess.Diagram(ctx, pts, lo, hi, density=w["density"], collect_stats=True)
+    ref = ds.cells_at_particles(sites, stats=True)
+    flops = float(ref["stats"]["flops"].astype(np.float64).sum())
+    faces = int(ref["face_count"].sum())
+    ds.close()
+    d = tess.Diagram(ctx, pts, lo, hi, density=w["density"])
+    d.compute()
+    lib = d.lib
+    ms2, rr = np.zeros(2), C.c_uint32()
+
+    def step():
+        h = C.c_void_p()
+        d.ctx.check(lib.tess_diagram_cells_at_particles(d.h, sites.ctypes.data_as(C.c_void_p), m, float("nan"), -1,
+                                                        C.byref(h)))
+        try:
+            d.ctx.check(lib.tess_site_cells_timings(h, ms2.ctypes.data_as(C.c_void_p), C.byref(rr)))
+        finally:
+            lib.tess_site_cells_destroy(h)
+        return dict(cells_ms=float(ms2[0]), compact_ms=float(ms2[1]), reruns=rr.value)
+
+    wall_ms, tims = timed(step, args.steps, args.warmup, ctx)
+    cells_ms = mean(tims, "cells_ms")
+    span_ms = cells_ms + mean(tims, "compact_ms")
+    achieved = flops / (cells_ms / 1e3) / 1e12
+    line = {
+        "metric": "Voronoi cells of outside sites/sec (get_cell_at_particle, f64, 1 MI355X)",
+        "value": m / (span_ms / 1e3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": span_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded xoshiro256**)",
+        "config": {"workload": w["desc"], "points": n, "sites": m, "cell_density": w["density"]},
+        "phases_ms": {"cells": cells_ms, "offsets_compact": mean(tims, "compact_ms")},
+        "whole_call": {"ms": wall_ms, "cells_per_s": m / (wall_ms / 1e3),
+                       "note": "host sites uploaded over PCIe and result buffers allocated inside the call"},
+        "capacity_reruns": tims[-1]["reruns"],
+        "faces": faces,
+        "roofline": {
+            "bound": "mfma",
+            "ceiling": "FP64 78.6 TFLOP/s (vector = matrix peak), FMA counted as 2; no MFMA in the clip",
+            "kernel": "cell passes k_cells<1, 2, 1, 0> + capacity rerun, HIP events on the context stream",
+            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops, "kernel_ms": cells_ms,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        threads = min(16, os.cpu_count() or 1)
+        k = min(m, int(2.5 * args.cpu_sample))  # a site costs about a fifth of a clustered cell
+        sel = np.random.default_rng(0).choice(m, k, replace=False)
+        t0 = time.perf_counter()
+        orc.compute_at_points(pts, sites[sel], lo, hi, density=w["density"], threads=threads)
+        sec = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": k / sec, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"{k} random sites of the {m} (seed 0 choice), grid build included, {sec:.1f} s on "
+                      f"{threads} threads of {os.cpu_count()} visible",
+        }
+    d.close()
+    print(json.dumps(line), flush=True)
+
+
 def bench_queries(ctx, w, args):
     """tess_diagram_query_device over HBM-resident query positions; one step
     is one call (count pass, scan, sizing, fill pass), results left in HBM."""
@@ -459,6 +549,9 @@ def main():
         return
     if w["kind"] == "geometry":
         bench_geometry(ctx, w, args)
+        return
+    if w["kind"] == "sites":
+        bench_sites(ctx, w, args)
         return
     fn = {"clustered": bench_clustered, "uniform": bench_uniform, "batch": bench_batch}[w["kind"]]
     pts, lo, hi, n, ms, tims, cells, extra, cfg_extra = fn(ctx, w, args)
