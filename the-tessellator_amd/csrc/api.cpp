@@ -336,6 +336,8 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   // work lists: one pass over every cell at the default capacity, or the
   // planner's lists, largest capacity first
   Schedule sch;
+  // geometry staging indexes the first pass by grid position: whole-set pass only
+  if (plan && d->geo_stage) return c->fail(TESS_E_STATE, "geometry staging needs the whole-set pass");
   if (plan) {
     if ((st = (*plan)(d, sch)) != TESS_OK) return st;
   } else {
