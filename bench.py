@@ -414,6 +414,17 @@ def bench_sites(ctx, w, args):
             "traffic": None,
         },
     }
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        # HBM bytes of the first cell pass over the sites, from the rocprofv3
+        # FETCH_SIZE / WRITE_SIZE runs of this workload, copy-kernel scaled
+        rec = json.load(open(tpath)).get(args.workload)
+        if rec:
+            line["roofline"]["traffic"] = rec["bytes_per_launch"]
+            line["roofline"]["traffic_kernel"] = rec["kernel"]
+    # algorithmic HBM bytes of the cell passes: site position (24 B), volume +
+    # face count + status (16 B), 12 B per staged face
+    line["roofline"]["alg_hbm_gbs"] = (40 * m + 12 * faces) / (cells_ms / 1e3) / 1e9
     if not args.no_cpu:
         from oracle import oracle as orc
 

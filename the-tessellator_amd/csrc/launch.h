@@ -103,6 +103,11 @@ hipError_t launch_morton(const double* x, const double* y, const double* z, int 
 hipError_t launch_index_maps(const uint32_t* codes, int n, uint32_t* iota, uint32_t* keys_out,
                              uint32_t* original_of_internal, uint32_t* internal_of_original, void* tmp,
                              size_t* tmp_bytes, hipStream_t s);
+// order = the sites (AoS xyz, m) stably sorted by their grid cell (clamped),
+// the work order of a pass over outside sites: neighbouring waves then read
+// the same grid cells.  tmp as launch_index_maps.
+hipError_t launch_site_order(const double* xyz, int m, const GridDev* g, uint32_t* keys, uint32_t* keys_out,
+                             uint32_t* iota, uint32_t* order, void* tmp, size_t* tmp_bytes, hipStream_t s);
 
 // mlp.hip: learned search-radius model (bf16 MFMA, fp32 accumulate)
 hipError_t launch_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);

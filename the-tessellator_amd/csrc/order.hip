@@ -51,6 +51,20 @@ __global__ void k_inverse(const uint32_t* __restrict__ perm, int n, uint32_t* __
   if (k < n) inv[perm[k]] = (uint32_t)k;
 }
 
+// Grid cell of each site (AoS xyz), clamped to the grid: sites inside the
+// container box but outside the points' bounding box take the nearest cell.
+__global__ void k_site_keys(const double* __restrict__ xyz, int m, const GridDev* __restrict__ g,
+                            uint32_t* __restrict__ key) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double top = (double)(g->cpd - 1);
+  const double fx = fmin(fmax(floor((xyz[3 * i] - g->xmin) * g->ix), 0.0), top);
+  const double fy = fmin(fmax(floor((xyz[3 * i + 1] - g->ymin) * g->iy), 0.0), top);
+  const double fz = fmin(fmax(floor((xyz[3 * i + 2] - g->zmin) * g->iz), 0.0), top);
+  const uint32_t c = (uint32_t)g->cpd;
+  key[i] = ((uint32_t)fz * c + (uint32_t)fy) * c + (uint32_t)fx;
+}
+
 inline unsigned blocks_of(int n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -75,6 +89,16 @@ hipError_t launch_index_maps(const uint32_t* codes, int n, uint32_t* iota, uint3
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_inverse, dim3(blocks_of(n)), dim3(256), 0, s, original_of_internal, n, internal_of_original);
   return hipGetLastError();
+}
+
+hipError_t launch_site_order(const double* xyz, int m, const GridDev* g, uint32_t* keys, uint32_t* keys_out,
+                             uint32_t* iota, uint32_t* order, void* tmp, size_t* tmp_bytes, hipStream_t s) {
+  if (!tmp)  // size query: LSD radix sort over the 32 key bits (stable)
+    return rocprim::radix_sort_pairs(nullptr, *tmp_bytes, keys, keys_out, iota, order, (size_t)m, 0, 32, s);
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_site_keys, dim3(blocks_of(m)), dim3(256), 0, s, xyz, m, g, keys);
+  hipLaunchKernelGGL(k_iota, dim3(blocks_of(m)), dim3(256), 0, s, iota, m);
+  return rocprim::radix_sort_pairs(tmp, *tmp_bytes, keys, keys_out, iota, order, (size_t)m, 0, 32, s);
 }
 
 }  // namespace tess

@@ -85,10 +85,26 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
       (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK)
     return st;
   CellArgs a = base_args(d, t);
+  // first pass in grid order of the sites, each XCD taking runs of them: waves
+  // that run together read the same grid cells (the outputs stay per site)
+  DevBuf<uint32_t> keys, keys_out, iota, order;
+  DevBuf<uint8_t> tmp;
+  size_t tmp_bytes = 0;
+  if ((st = c->hip(launch_site_order(nullptr, (int)m, a.grid, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                     &tmp_bytes, s),
+                   "sort")) != TESS_OK)
+    return st;
+  if (keys.alloc(m) != hipSuccess || keys_out.alloc(m) != hipSuccess || iota.alloc(m) != hipSuccess ||
+      order.alloc(m) != hipSuccess || tmp.alloc(tmp_bytes) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if ((st = c->hip(launch_site_order(p->sites.p, (int)m, a.grid, keys.p, keys_out.p, iota.p, order.p, tmp.p,
+                                     &tmp_bytes, s),
+                   "sort")) != TESS_OK)
+    return st;
   a.ext = p->sites.p;
-  a.work = nullptr;
+  a.work = order.p;
   a.nwork = (int)m;
-  a.xcd_chunk = 0;
+  a.xcd_chunk = plain_xcd_chunk(d);
   a.target_group = target_group;
   a.radius_limit = search_radius;
   a.volume = p->volume.p;
@@ -113,6 +129,7 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
     if (p->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
     a.work = p->redo_list.p;
     a.nwork = (int)nredo;
+    a.xcd_chunk = 0;
     a.redo = nullptr;
     a.nredo = nullptr;
     if ((st = c->hip(hipMemcpyAsync(p->redo_list.p, p->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
