@@ -32,10 +32,11 @@ def _sites(pts, seed, m):
     return s.clip(0, 1)
 
 
+@pytest.mark.parametrize("m", [3000, 20000])  # input order; grid order (4096 sites and more)
 @pytest.mark.parametrize("workload", ["uniform", "clustered"])
-def test_sites_match_oracle(ctx, workload):
+def test_sites_match_oracle(ctx, workload, m):
     pts = tess.uniform(30000, 61) if workload == "uniform" else tess.clustered(30000, 62)
-    sites = _sites(pts, 63, 3000)
+    sites = _sites(pts, 63, m)
     d = tess.Diagram(ctx, pts, LO, HI, density=5.0, collect_stats=True)
     got = d.cells_at_particles(sites, stats=True)
     want = orc.compute_at_points(pts, sites, LO, HI, density=5.0)

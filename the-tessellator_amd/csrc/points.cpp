@@ -28,6 +28,9 @@ struct tess_site_cells {
 
 namespace {
 
+// Site sets from this size on are taken in grid order.
+constexpr size_t kSiteOrderMin = 4096;
+
 struct Events {
   hipEvent_t e[3] = {nullptr, nullptr, nullptr};
   Events() {
@@ -85,26 +88,31 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
       (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK)
     return st;
   CellArgs a = base_args(d, t);
-  // first pass in grid order of the sites, each XCD taking runs of them: waves
-  // that run together read the same grid cells (the outputs stay per site)
+  a.ext = p->sites.p;
+  a.work = nullptr;
+  a.nwork = (int)m;
+  a.xcd_chunk = 0;
+  // larger sets: first pass in grid order of the sites, each XCD taking runs
+  // of them, so that waves running together read the same grid cells (the
+  // outputs stay per site); small sets are not worth the sort's launches
   DevBuf<uint32_t> keys, keys_out, iota, order;
   DevBuf<uint8_t> tmp;
-  size_t tmp_bytes = 0;
-  if ((st = c->hip(launch_site_order(nullptr, (int)m, a.grid, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                     &tmp_bytes, s),
-                   "sort")) != TESS_OK)
-    return st;
-  if (keys.alloc(m) != hipSuccess || keys_out.alloc(m) != hipSuccess || iota.alloc(m) != hipSuccess ||
-      order.alloc(m) != hipSuccess || tmp.alloc(tmp_bytes) != hipSuccess)
-    return c->fail(TESS_E_NOMEM, "device allocation failed");
-  if ((st = c->hip(launch_site_order(p->sites.p, (int)m, a.grid, keys.p, keys_out.p, iota.p, order.p, tmp.p,
-                                     &tmp_bytes, s),
-                   "sort")) != TESS_OK)
-    return st;
-  a.ext = p->sites.p;
-  a.work = order.p;
-  a.nwork = (int)m;
-  a.xcd_chunk = plain_xcd_chunk(d);
+  if (m >= kSiteOrderMin) {
+    size_t tmp_bytes = 0;
+    if ((st = c->hip(launch_site_order(nullptr, (int)m, a.grid, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                       &tmp_bytes, s),
+                     "sort")) != TESS_OK)
+      return st;
+    if (keys.alloc(m) != hipSuccess || keys_out.alloc(m) != hipSuccess || iota.alloc(m) != hipSuccess ||
+        order.alloc(m) != hipSuccess || tmp.alloc(tmp_bytes) != hipSuccess)
+      return c->fail(TESS_E_NOMEM, "device allocation failed");
+    if ((st = c->hip(launch_site_order(p->sites.p, (int)m, a.grid, keys.p, keys_out.p, iota.p, order.p, tmp.p,
+                                       &tmp_bytes, s),
+                     "sort")) != TESS_OK)
+      return st;
+    a.work = order.p;
+    a.xcd_chunk = plain_xcd_chunk(d);
+  }
   a.target_group = target_group;
   a.radius_limit = search_radius;
   a.volume = p->volume.p;
