@@ -549,8 +549,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="clustered4m: also time a fixed set of batch configurations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path; --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
 
     w = WORKLOADS[args.workload]
