@@ -470,12 +470,17 @@ def bench_queries(ctx, w, args):
     ms, tims = timed(step, args.steps, args.warmup, ctx)
     fill_ms = mean(tims, "fill_ms")
     count_ms = mean(tims, "count_ms")
-    # algorithmic bytes of the fill pass: per query the position (24 B) and its
-    # two CSR offsets (16 B); per in-range grid cell its two delimiters (8 B);
-    # per candidate its sorted index and gathered position (28 B); per kept
-    # point its id (4 B)
-    fill_bytes = 40 * n + 8 * cells + 28 * cands + 4 * kept
+    # algorithmic (compulsory) bytes of the fill pass, every datum once: per
+    # query its position (24 B), its slot offset (8 B) and its kept count
+    # (4 B); per point its sorted index and position (28 B); the grid's
+    # delimiters (4 B per cell); per kept point its id (4 B)
+    cpd = d.grid()[2]
+    fill_bytes = 36 * n + 28 * n + 4 * (cpd ** 3 + 1) + 4 * kept
     achieved = fill_bytes / (fill_ms / 1e3) / 1e9
+    # the same reads counted per query (every in-range grid cell's two
+    # delimiters, every candidate's index and position): mostly served by the
+    # caches, since neighbouring queries share grid cells
+    per_query_bytes = 40 * n + 8 * cells + 28 * cands + 4 * kept
     line = {
         "metric": "radius queries/sec (find_neighbors_in_real_radius, f64, 1 MI355X)",
         "value": n / (ms / 1e3),
@@ -502,15 +507,16 @@ def bench_queries(ctx, w, args):
             "bytes_per_launch": fill_bytes,
             "kernel_ms": fill_ms,
             "traffic": None,
+            "per_query_bytes": per_query_bytes,
+            "per_query_gbs": per_query_bytes / (fill_ms / 1e3) / 1e9,
         },
     }
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
         rec = json.load(open(tpath)).get(args.workload)
         if rec:
-            # the per-query bytes count every query's own reads; neighbouring
-            # queries share grid cells, so the HBM bytes are fewer and the
-            # algorithmic rate may exceed the HBM peak
+            # measured HBM bytes: above the compulsory ones by the kept ids'
+            # partial-line writes and the re-reads of shared grid cells
             line["roofline"]["traffic"] = rec["bytes_per_launch"]
             line["roofline"]["traffic_kernel"] = rec["kernel"]
             line["roofline"]["hbm_gbs"] = rec["bytes_per_launch"] / (fill_ms / 1e3) / 1e9
