@@ -612,7 +612,15 @@ def main():
         # FETCH_SIZE / WRITE_SIZE runs of this workload, copy-kernel scaled
         rec = json.load(open(tpath)).get(args.workload)
         if rec:
-            line["roofline"]["traffic"] = rec["bytes_per_launch"]
+            traffic = rec["bytes_per_launch"]
+            byx = rec.get("by_xcd_chunk")
+            tuned = extra.get("tuned_config")
+            if byx and tuned:
+                # the figure measured under the XCD mapping this run was timed
+                # with (none if that mapping was not measured)
+                traffic = byx.get(str(tuned["xcd_chunk"]))
+                line["roofline"]["traffic_config"] = f"xcd_chunk {tuned['xcd_chunk']}"
+            line["roofline"]["traffic"] = traffic
             line["roofline"]["traffic_kernel"] = rec["kernel"]
     if faces is not None:
         # algorithmic HBM bytes of the cell passes: site position + id (28 B),
