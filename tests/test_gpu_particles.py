@@ -67,10 +67,11 @@ def test_site_on_removed_point_is_that_points_cell(ctx):
     assert checked >= len(idx) // 2
 
 
-def test_sites_with_radius_and_group(ctx):
+@pytest.mark.parametrize("m", [500, 5000])  # input order; grid order
+def test_sites_with_radius_and_group(ctx, m):
     pts = tess.clustered(20000, 65)
     groups = np.random.default_rng(66).integers(0, 3, len(pts)).astype(np.int64)
-    sites = _sites(pts, 67, 500)
+    sites = _sites(pts, 67, m)
     d = tess.Diagram(ctx, pts, LO, HI, groups=groups, density=5.0, collect_stats=True)
     for r, g in ((0.002, None), (None, 2), (0.01, 0)):
         got = d.cells_at_particles(sites, search_radius=r, target_group=g, stats=True)
