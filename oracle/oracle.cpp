@@ -1406,6 +1406,15 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
                                   const double lo[3], const double hi[3], double density, int mode, int clip,
                                   const double* sites, size_t nsites, int nthreads, double max_radius,
                                   const int64_t* groups, int64_t target_group) {
+  return orc_compute_at_points_ordered(x, y, z, n, lo, hi, density, mode, clip, sites, nsites, nthreads, max_radius,
+                                       groups, target_group, 0);
+}
+
+orc_result* orc_compute_at_points_ordered(const double* x, const double* y, const double* z, size_t n,
+                                          const double lo[3], const double hi[3], double density, int mode,
+                                          int clip, const double* sites, size_t nsites, int nthreads,
+                                          double max_radius, const int64_t* groups, int64_t target_group,
+                                          int order) {
   Filter flt;
   flt.max_radius = max_radius;
   flt.groups = groups;
@@ -1414,7 +1423,7 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
   if (n == 0) return R;
   auto t0 = std::chrono::steady_clock::now();
   Grid G;
-  grid_setup(G, x, y, z, n, density, mode == ORC_MODE_ALL_POINTS);
+  grid_setup(G, x, y, z, n, density, mode == ORC_MODE_ALL_POINTS, order);
   const size_t ncells = nsites;
   R->ncells = ncells;
   R->volume.resize(ncells);

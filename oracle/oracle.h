@@ -169,6 +169,13 @@ orc_result* orc_compute_at_points(const double* x, const double* y, const double
                                   const double lo[3], const double hi[3], double density, int mode, int clip,
                                   const double* sites, size_t nsites, int nthreads, double max_radius,
                                   const int64_t* groups, int64_t target_group);
+/* orc_compute_at_points with the candidate order of a grid cell chosen by
+ * order (as orc_compute_ordered). */
+orc_result* orc_compute_at_points_ordered(const double* x, const double* y, const double* z, size_t n,
+                                          const double lo[3], const double hi[3], double density, int mode,
+                                          int clip, const double* sites, size_t nsites, int nthreads,
+                                          double max_radius, const int64_t* groups, int64_t target_group,
+                                          int order);
 /* orc_compute (security stop, per-face clip) plus the geometry of every cell:
  * the vertex ring of each face (compute_face_vertices, polyhedron.rs:895-919),
  * faces in output order, each ring from its lexicographically smallest

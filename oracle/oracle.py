@@ -92,6 +92,8 @@ def lib():
                                        C.c_double, dp, C.c_int64]),
             "orc_compute_at_points": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz, C.c_int,
                                            C.c_double, dp, C.c_int64]),
+            "orc_compute_at_points_ordered": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, C.c_int, C.c_int, dp, sz,
+                                                   C.c_int, C.c_double, dp, C.c_int64, C.c_int]),
             "orc_compute_geometry": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, dp, sz, C.c_int]),
             "orc_compute_geometry_ordered": (vp, [dp, dp, dp, sz, dp, dp, C.c_double, dp, sz, C.c_int, C.c_int]),
             "orc_result_ring_entries": (C.c_uint64, [vp]),
@@ -327,16 +329,18 @@ def compute(xyz, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, cell
 
 
 def compute_at_points(xyz, sites, lo, hi, density=1.25, mode=MODE_SECURITY, clip=CLIP_FACES, threads=0,
-                      max_radius=None, groups=None, target_group=None):
-    """Cells of sites that are not points of the set (get_cell_at_particle), same dict as compute."""
+                      max_radius=None, groups=None, target_group=None, order=0):
+    """Cells of sites that are not points of the set (get_cell_at_particle), same dict as compute;
+    order as compute()."""
     x, y, z = soa(xyz)
     n = len(x)
     lo, hi = _v(lo), _v(hi)
     st = np.ascontiguousarray(np.asarray(sites, np.float64).reshape(-1, 3))
     g = None if groups is None else np.ascontiguousarray(groups, np.int64)
-    r = lib().orc_compute_at_points(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(st), len(st),
-                                    threads, float("nan") if max_radius is None else float(max_radius), _p(g),
-                                    -1 if target_group is None else int(target_group))
+    r = lib().orc_compute_at_points_ordered(_p(x), _p(y), _p(z), n, _p(lo), _p(hi), density, mode, clip, _p(st),
+                                            len(st), threads,
+                                            float("nan") if max_radius is None else float(max_radius), _p(g),
+                                            -1 if target_group is None else int(target_group), order)
     return _result(r, len(st))
 
 

@@ -97,3 +97,18 @@ def test_morton_order_geometry_matches_oracle(ctx, computed):
     entry_cell = np.repeat(np.repeat(np.arange(len(fc)), fc), np.diff(geo["ring_offset"].astype(np.int64)))
     xyz = geo["vertices"][geo["vertex_offset"][entry_cell].astype(np.int64) + geo["ring"]]
     assert np.array_equal(xyz, ref["ring_xyz"])
+
+
+@pytest.mark.parametrize("m", [3000, 20000])  # sites in input order; in grid order
+def test_morton_order_sites_match_oracle(ctx, m):
+    """Cells of outside sites over a diagram in the internal order, bitwise
+    against the oracle in that order, counters included."""
+    pts = tess.clustered(20000, 98)
+    sites = np.random.default_rng(99).uniform(0, 1, (m, 3))
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0, point_order=tess.ORDER_MORTON, collect_stats=True)
+    got = d.cells_at_particles(sites, stats=True)
+    ref = orc.compute_at_points(pts, sites, LO, HI, density=5.0, order=orc.ORDER_MORTON)
+    for k in ("volume", "face_count", "face_offset", "neighbor", "area"):
+        assert np.array_equal(got[k], ref[k]), k
+    for k in ("streamed", "classified", "cuts", "entries", "flops"):
+        assert np.array_equal(got["stats"][k], ref["stats"][k]), k

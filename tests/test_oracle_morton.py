@@ -95,3 +95,17 @@ def test_geometry_in_internal_order():
         ring = g["ring_xyz"][ro[f]:ro[f + 1]]
         w = sum(np.cross(ring[k] - ring[0], ring[k + 1] - ring[0]) for k in range(1, len(ring) - 1))
         assert abs(0.5 * np.linalg.norm(w) - g["area"][f]) <= 1e-12 * max(g["area"][f], 1e-12)
+
+
+def test_sites_in_internal_order():
+    """Cells of outside sites over a grid in the internal order: the same
+    neighbours as in input order, volumes to 1e-12, and the order reaches the
+    clip (some volume differs in its last bits)."""
+    pts = tess.uniform(3000, 36)
+    sites = np.random.default_rng(37).uniform(0, 1, (500, 3))
+    lo, hi = np.zeros(3), np.ones(3)
+    a = orc.compute_at_points(pts, sites, lo, hi, density=5.0)
+    b = orc.compute_at_points(pts, sites, lo, hi, density=5.0, order=orc.ORDER_MORTON)
+    assert np.array_equal(a["neighbor"], b["neighbor"])
+    assert np.allclose(a["volume"], b["volume"], rtol=1e-12, atol=0)
+    assert not np.array_equal(a["volume"], b["volume"])
