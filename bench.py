@@ -348,6 +348,20 @@ def bench_geometry(ctx, w, args):
             "traffic": None,
         },
     }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        threads = min(16, os.cpu_count() or 1)
+        k = min(n, args.cpu_sample)
+        cells = np.sort(np.random.default_rng(0).choice(n, k, replace=False)).astype(np.uint32)
+        t0 = time.perf_counter()
+        orc.compute_geometry(pts, np.zeros(3), np.ones(3), density=w["density"], cells=cells, threads=threads)
+        sec = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": k / sec, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"{k} random cells of the {n} (seed 0 choice) with their rings, cells and grid build "
+                      f"included, {sec:.1f} s on {threads} threads of {os.cpu_count()} visible",
+        }
     d.close()
     print(json.dumps(line), flush=True)
 
