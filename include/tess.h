@@ -88,6 +88,11 @@ typedef struct {
   double r2;           /* final max |vertex - site|^2 */
   uint32_t max_vertices;
   uint32_t status;     /* 0 ok; TESS_CELL_* bits */
+  uint32_t incident_cuts; /* cuts whose Outside region touched no Inside vertex
+                             (bordered by Incident vertices only): the
+                             reference's find_outgoing_edge, polyhedron.rs:418-432,
+                             needs an Inside -> Outside edge and skips them */
+  uint32_t reserved;
 } tess_cell_stats;
 
 enum {
@@ -95,7 +100,15 @@ enum {
   TESS_CELL_TABLE = 2u,      /* shell table exhausted before the stop */
   TESS_CELL_DEGENERATE = 4u, /* topology check failed */
   TESS_CELL_FALLBACK = 8u,   /* learned mode: prediction under-sized the batch */
-  TESS_CELL_RERUN = 16u      /* recomputed at the largest capacity after overflowing */
+  TESS_CELL_RERUN = 16u,     /* recomputed at the largest capacity after overflowing */
+  TESS_CELL_LONG_TABLE = 32u,/* recomputed with a longer shell table: its security
+                                ball reached past the default 25^3-offset prefix
+                                (the reference streams all (2 cpd - 1)^3 offsets,
+                                celery.rs:418-679, :971-1018) */
+  TESS_CELL_SPILL = 64u      /* more than 256 vertices or 128 faces: computed by
+                                the global-memory kernel (no size limit but the
+                                workspace, grown on demand like the reference's
+                                Pool, pool.rs:85-110) */
 };
 
 /* Device pointers to a computed diagram's results (valid until destroy). */
@@ -120,6 +133,10 @@ typedef struct {
   uint64_t fallback_cells; /* learned mode: cells that fell back to the plain search */
   uint64_t capacity_reruns;/* cells rerun with a larger capacity */
   double plan_ms;          /* learned mode: features + inference + work lists */
+  uint64_t table_reruns;   /* cells recomputed with a longer shell table */
+  uint64_t spill_cells;    /* cells computed by the global-memory spill kernel */
+  uint64_t incident_cuts;  /* cuts bordered by Incident vertices only (see
+                              tess_cell_stats.incident_cuts) */
 } tess_timings;
 
 void tess_opts_default(tess_opts* opts);
@@ -370,6 +387,9 @@ typedef struct {
                              K > 0: in runs of K consecutive entries */
   int32_t redo_chunk;     /* cells a wavefront takes per grab in the rerun pass */
   int32_t learned;        /* 1: heaviest predicted cells first (needs a model) */
+  int32_t eager;          /* 1: launch the passes directly on the stream
+                             instead of capturing and replaying HIP graphs
+                             (same kernels, same arguments) */
 } tess_batch_config;
 
 /* xyz[s]: host AoS points of set s (n[s] of them); lo/hi: [nsets][3] boxes. */

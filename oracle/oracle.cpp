@@ -1090,6 +1090,7 @@ void compute_cell(Grid& G, const double* x, const double* y, const double* z, si
         s.cuts++;
         int nvv = C.nv();
         flops += 12ull * (cs.new_vertices - nv0) + 5ull * nvv;
+        if (cs.new_vertices == nv0) s.incident_cuts++;  // no Inside-Outside edge was crossed
         r2 = C.r2();
         if (nvv > s.max_vertices) s.max_vertices = nvv;
         if (C.nf() > s.max_faces) s.max_faces = C.nf();

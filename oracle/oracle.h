@@ -150,6 +150,10 @@ typedef struct {
   int32_t max_faces;
   int32_t max_edges;    /* largest live half-edge count reached */
   int32_t status;       /* 0 ok, nonzero = topology failure */
+  int32_t incident_cuts;/* cuts that created no intersection vertex: the Outside
+                           region was bordered by Incident vertices only, which
+                           polyhedron.rs:418-432 (find_outgoing_edge) skips */
+  int32_t reserved;
 } orc_cell_stats;
 
 typedef struct orc_result orc_result;

@@ -28,12 +28,14 @@ class Grid(C.Structure):
 class CellStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("streamed", "classified", "cuts", "new_vertices", "entries", "flops")] + \
                [("r2", C.c_double), ("stop_key", C.c_double)] + \
-               [(k, C.c_int32) for k in ("max_vertices", "max_faces", "max_edges", "status")]
+               [(k, C.c_int32) for k in ("max_vertices", "max_faces", "max_edges", "status", "incident_cuts",
+                                         "reserved")]
 
 
 STATS_DTYPE = np.dtype([(k, "<u8") for k in ("streamed", "classified", "cuts", "new_vertices", "entries", "flops")] +
                        [("r2", "<f8"), ("stop_key", "<f8")] +
-                       [(k, "<i4") for k in ("max_vertices", "max_faces", "max_edges", "status")])
+                       [(k, "<i4") for k in ("max_vertices", "max_faces", "max_edges", "status", "incident_cuts",
+                                             "reserved")])
 assert STATS_DTYPE.itemsize == C.sizeof(CellStats)
 
 _lib = None

@@ -39,7 +39,7 @@ def test_morton_order_cells_match_oracle(ctx, workload, density):
     for k in ("volume", "face_count", "face_offset", "neighbor", "area"):
         assert np.array_equal(got[k], ref[k]), k
     st = d.stats()
-    for k in ("streamed", "classified", "cuts", "entries", "flops"):
+    for k in ("streamed", "classified", "cuts", "entries", "flops", "incident_cuts"):
         assert np.array_equal(st[k], ref["stats"][k]), k
     _, srt, dl = d.copy_grid()
     _, _, r_srt, r_dl = orc.grid(pts, density, order=orc.ORDER_MORTON)
@@ -110,5 +110,5 @@ def test_morton_order_sites_match_oracle(ctx, m):
     ref = orc.compute_at_points(pts, sites, LO, HI, density=5.0, order=orc.ORDER_MORTON)
     for k in ("volume", "face_count", "face_offset", "neighbor", "area"):
         assert np.array_equal(got[k], ref[k]), k
-    for k in ("streamed", "classified", "cuts", "entries", "flops"):
+    for k in ("streamed", "classified", "cuts", "entries", "flops", "incident_cuts"):
         assert np.array_equal(got["stats"][k], ref["stats"][k]), k

@@ -88,5 +88,21 @@ def test_uniform_cells_match_oracle(ctx, n, seed, density):
     _compare(got, ref)
     assert abs(got["volume"].sum() - 1.0) <= 1e-10
     st, rs = d.stats(), ref["stats"]
-    for k in ("streamed", "classified", "cuts", "entries", "flops"):
+    for k in ("streamed", "classified", "cuts", "entries", "flops", "incident_cuts"):
         assert np.array_equal(st[k].astype(np.uint64), rs[k].astype(np.uint64)), k
+
+
+def test_incident_only_cuts_counted(ctx):
+    """Cuts bordered by Incident vertices only (the reference's
+    find_outgoing_edge skips them, polyhedron.rs:418-432): counted per cell
+    exactly as the oracle counts them, and the cells exact."""
+    g = (np.arange(4) + 0.5) / 4
+    pts = np.array([(x, y, z) for x in g for y in g for z in g])
+    d = tess.Diagram(ctx, pts, UNIT_LO, UNIT_HI, collect_stats=True)
+    d.compute()
+    ref = orc.compute(pts, UNIT_LO, UNIT_HI)
+    st = d.stats()
+    assert np.array_equal(st["incident_cuts"].astype(np.int64), ref["stats"]["incident_cuts"].astype(np.int64))
+    assert st["incident_cuts"].sum() > 0
+    assert d.timings()["incident_cuts"] == st["incident_cuts"].sum()
+    assert np.abs(d.cells()["volume"] - 1 / 64).max() < 1e-15

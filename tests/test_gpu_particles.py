@@ -19,7 +19,7 @@ def _same(got, want, stats=True):
     for k in ("volume", "face_count", "face_offset", "neighbor", "area"):
         assert np.array_equal(got[k], want[k]), k
     if stats:
-        for k in ("streamed", "classified", "cuts", "entries", "flops"):
+        for k in ("streamed", "classified", "cuts", "entries", "flops", "incident_cuts"):
             assert np.array_equal(got["stats"][k], want["stats"][k]), k
 
 

@@ -153,3 +153,23 @@ def test_poisson_mean_face_count():
         nb, _ = csr(r, i)
         fc.append(int((nb >= 0).sum()))
     assert abs(np.mean(fc) - (2 + 48 * np.pi ** 2 / 35)) < 0.08
+
+
+def test_incident_only_cuts_counted():
+    """A simple cubic lattice: the bisectors pass through vertices, so some
+    cuts cut off a region bordered by Incident vertices only.  The reference's
+    find_outgoing_edge (polyhedron.rs:418-432) needs an Inside -> Outside edge
+    and skips them -- the walk clip, which keeps that rule, counts none and
+    gets the wrong cells -- while the per-face clip cuts them (exact 1/64
+    volumes) and counts each one."""
+    g = (np.arange(4) + 0.5) / 4
+    pts = np.array([(x, y, z) for x in g for y in g for z in g])
+    faces = orc.compute(pts, LO, HI, clip=orc.CLIP_FACES)
+    walk = orc.compute(pts, LO, HI, clip=orc.CLIP_WALK)
+    assert faces["stats"]["incident_cuts"].sum() > 0
+    assert np.abs(faces["volume"] - 1 / 64).max() < 1e-15
+    assert walk["stats"]["incident_cuts"].sum() == 0
+    assert np.abs(walk["volume"] - 1 / 64).max() > 1e-3
+    # uniform points in general position never cut that way
+    u = orc.compute(tess.uniform(2000, 5), LO, HI)
+    assert u["stats"]["incident_cuts"].sum() == 0

@@ -132,7 +132,11 @@ tess_status alloc_all(tess_diagram* d) {
             d->redo.alloc(n + 1) == hipSuccess && d->snbr.alloc(n * kFaceStage) == hipSuccess &&
             d->sarea.alloc(n * kFaceStage) == hipSuccess && d->foff.alloc(n + 1) == hipSuccess &&
             d->fsums.alloc((n + 1023) / 1024 + 1) == hipSuccess && d->work1.alloc(1) == hipSuccess &&
-            d->summary.alloc(1) == hipSuccess;
+            d->summary.alloc(1) == hipSuccess && d->tredo.alloc(n + 1) == hipSuccess &&
+            d->spill.alloc(n + 1) == hipSuccess && d->fpos.alloc(n) == hipSuccess &&
+            d->arena_ctr.alloc(1) == hipSuccess;
+  d->arena_next = d->arena_ctr.p;
+  d->arena_cap = (uint64_t)n * kFaceStage;
   if (ok && d->opts.collect_stats) ok = d->stats.alloc(n) == hipSuccess;
   if (!ok) return c->fail(TESS_E_NOMEM, "device allocation failed");
   return TESS_OK;
@@ -220,6 +224,9 @@ CellArgs base_args(tess_diagram* d, const Table* t) {
   a.fcount = d->fcount.p;
   a.snbr = d->snbr.p;
   a.sarea = d->sarea.p;
+  a.fpos = d->fpos.p;
+  a.arena_next = d->arena_next;
+  a.arena_cap = d->arena_cap;
   a.status = d->status.p;
   a.stats = d->opts.collect_stats ? d->stats.p : nullptr;
   a.summary = d->summary.p;
@@ -280,8 +287,14 @@ tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, 
   a.nwork = nwork;
   // whole set in grid order: each XCD takes runs of consecutive cells
   a.xcd_chunk = work ? 0 : plain_xcd_chunk(d);
-  a.redo = push_redo ? d->redo.p + 1 : nullptr;
-  a.nredo = push_redo ? d->redo.p : nullptr;
+  // capacity overflows: below 256 vertices to the rerun list, at 256 to the
+  // spill list; shell-table exhaustions to their own list
+  DevBuf<uint32_t>& cap = vpl < 4 ? d->redo : d->spill;
+  a.redo = push_redo ? cap.p + 1 : nullptr;
+  a.nredo = push_redo ? cap.p : nullptr;
+  a.tredo = d->tredo.p + 1;
+  a.ntredo = d->tredo.p;
+  a.defer_table = 1;
   if (GeoStage* g = d->geo_stage) {  // the same pass, staging the geometry too
     a.geo_count = g->count;
     GeoStaging& b = work ? g->rerun : g->first;
@@ -345,19 +358,18 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   }
   (void)hipEventRecord(ev.e[2], s);
   if ((st = c->hip(hipMemsetAsync(d->summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK ||
-      (st = c->hip(hipMemsetAsync(d->redo.p, 0, sizeof(uint32_t), s), "memset")) != TESS_OK)
+      (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(d->tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(d->spill.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32((uint32_t*)d->arena_next, 2, 0u, s), "fill")) != TESS_OK)
     return st;
   static const int kVpl[3] = {1, 2, 4};
   for (int slot = 2; slot >= 0; --slot)
-    if (sch.count[slot] &&
-        (st = run_cells(d, t, kVpl[slot], sch.list[slot], sch.count[slot], slot < 2)) != TESS_OK)
+    if (sch.count[slot] && (st = run_cells(d, t, kVpl[slot], sch.list[slot], sch.count[slot], true)) != TESS_OK)
       return st;
   // cells that outgrew their capacity: rerun with the largest variant
   uint32_t nredo = 0;
-  Summary sm1;
   if ((st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&sm1, d->summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
-          TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
     return st;
   if (nredo) {
@@ -368,25 +380,36 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
     }
     if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
                      "redo")) != TESS_OK ||
-        (st = c->hip(hipMemsetAsync(&d->summary.p->capacity, 0, sizeof(unsigned int), s), "memset")) != TESS_OK ||
-        (st = run_cells(d, t, 4, d->redo_list.p, (int)nredo, false)) != TESS_OK)
-      return st;
-    if ((st = c->hip(launch_mark_status(d->redo_list.p, (int)nredo, d->sorted.p, d->status.p,
-                                        d->opts.collect_stats ? d->stats.p : nullptr,
-                                        TESS_CELL_RERUN | (plan ? TESS_CELL_FALLBACK : 0u), s),
-                     "mark")) != TESS_OK)
+        (st = run_cells(d, t, 4, d->redo_list.p, (int)nredo, true)) != TESS_OK)
       return st;
   }
+  // cells past the shell table or the largest LDS capacity
+  std::vector<uint32_t> tab, spl;
+  if ((st = read_list(c, d->tredo.p, tab)) != TESS_OK || (st = read_list(c, d->spill.p, spl)) != TESS_OK) return st;
+  FinishCounts fin;
+  if (!tab.empty() || !spl.empty()) {
+    CellArgs a = base_args(d, t);
+    if ((st = finish_cells(d, a, std::move(tab), std::move(spl), &fin, d->geo_stage)) != TESS_OK) return st;
+  }
+  if (nredo && (st = c->hip(launch_mark_status(d->redo_list.p, (int)nredo, d->sorted.p, d->status.p,
+                                               d->opts.collect_stats ? d->stats.p : nullptr,
+                                               TESS_CELL_RERUN | (plan ? TESS_CELL_FALLBACK : 0u), s),
+                            "mark")) != TESS_OK)
+    return st;
   (void)hipEventRecord(ev.e[3], s);
   if ((st = c->hip(launch_face_offsets(d->fcount.p, n, d->foff.p, d->fsums.p, s), "face offsets")) != TESS_OK)
     return st;
   uint64_t total = 0;
+  unsigned long long used = 0;
   if ((st = c->hip(hipMemcpyAsync(&total, d->foff.p + n, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&used, d->arena_next, 8, hipMemcpyDeviceToHost, s), "arena")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK)
     return st;
+  if (used > d->arena_cap) return c->fail(TESS_E_CAPACITY, "the face arena is full");
   if (d->nbr.alloc(total) != hipSuccess || d->area.alloc(total) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  if ((st = c->hip(launch_compact(d->fcount.p, d->foff.p, n, d->snbr.p, d->sarea.p, d->nbr.p, d->area.p, s),
+  if ((st = c->hip(launch_compact(d->fcount.p, d->foff.p, n, d->snbr.p, d->sarea.p, d->nbr.p, d->area.p, s,
+                                  d->fpos.p),
                    "compact")) != TESS_OK)
     return st;
   (void)hipEventRecord(ev.e[4], s);
@@ -404,15 +427,15 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   d->tim.capacity_reruns = nredo;
   d->tim.fallback_cells = plan ? nredo : 0;
   d->tim.flops = sm2.flops;
+  d->tim.incident_cuts = sm2.incident_cuts;
+  d->table_reruns = d->tim.table_reruns = fin.table;
+  d->spill_cells = d->tim.spill_cells = fin.spill;
   d->total_faces = total;
   d->computed = true;
-  // every first-pass overflow that went to the redo list is counted again
-  // (or not) by the rerun; overflows of the largest variant are failures
-  const uint64_t cap = nredo ? (uint64_t)(sm1.capacity - nredo) + sm2.capacity : sm1.capacity;
-  const uint64_t bad = sm2.degenerate, tab = sm2.table;
+  const uint64_t bad = sm2.degenerate;
   d->degenerate = bad;
-  if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
-  if (tab) return c->fail(TESS_E_CAPACITY, std::to_string(tab) + " cells exhausted the shell table");
+  if (fin.failed)
+    return c->fail(TESS_E_CAPACITY, std::to_string(fin.failed) + " cells exceeded the spill workspace limit");
   if (bad) return c->fail(TESS_E_DEGENERATE, std::to_string(bad) + " cells failed topology checks");
   return TESS_OK;
 }
@@ -697,28 +720,39 @@ tess_status single_cell(tess_diagram* d, size_t index, const double* site, doubl
   if (d->exp_v.alloc(3 * 256) != hipSuccess || d->exp_meta.alloc(2 + 256) != hipSuccess ||
       d->exp_ring.alloc(kFaceStage * 64) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  if (site && (d->one_site.alloc(3) != hipSuccess || d->one_vol.alloc(1) != hipSuccess ||
-               d->one_u32.alloc(2) != hipSuccess || d->one_nbr.alloc(kFaceStage) != hipSuccess ||
-               d->one_area.alloc(kFaceStage) != hipSuccess))
-    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  if (site && d->one_site.alloc(3) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
   const GridDev& G = d->hgrid;
-  Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, kTableL, &st);
-  if (!t) return st;
-  // outputs: the diagram's slot of the point, or the single-site buffers
-  const uint32_t oi = site ? 0u : (uint32_t)index;
-  double* vol = site ? d->one_vol.p : d->volume.p;
-  uint32_t* fcount = site ? d->one_u32.p : d->fcount.p;
-  uint32_t* status_p = site ? d->one_u32.p + 1 : d->status.p;
-  int32_t* snbr = site ? d->one_nbr.p : d->snbr.p;
-  double* sarea = site ? d->one_area.p : d->sarea.p;
+  // the cell's outputs go to buffers of its own (the diagram's results stay
+  // as they are); the kernel indexes them by the point's original index, so
+  // the point's cell gets bases shifted back by that index
+  if (d->one_vol.alloc(1) != hipSuccess || d->one_u32.alloc(2) != hipSuccess ||
+      d->one_nbr.alloc(kFaceStage) != hipSuccess || d->one_area.alloc(kFaceStage) != hipSuccess)
+    return c->fail(TESS_E_NOMEM, "device allocation failed");
+  const size_t shift = site ? 0 : index;
+  auto shifted = [shift](auto* p, size_t per) {
+    return reinterpret_cast<decltype(p)>(reinterpret_cast<uintptr_t>(p) - shift * per);
+  };
+  double* vol = shifted(d->one_vol.p, sizeof(double));
+  uint32_t* fcount = shifted(d->one_u32.p, sizeof(uint32_t));
+  uint32_t* status_p = shifted(d->one_u32.p + 1, sizeof(uint32_t));
+  int32_t* snbr = shifted(d->one_nbr.p, kFaceStage * sizeof(int32_t));
+  double* sarea = shifted(d->one_area.p, kFaceStage * sizeof(double));
+  const uint32_t oi = (uint32_t)shift;
   if (site && (st = c->hip(hipMemcpyAsync(d->one_site.p, site, 24, hipMemcpyHostToDevice, s), "upload")) != TESS_OK)
     return st;
-  uint32_t tries[2] = {2, 4};
   std::unique_ptr<tess_cell> cell(new (std::nothrow) tess_cell);
   if (!cell) return c->fail(TESS_E_NOMEM, "host allocation failed");
   uint32_t status = 0;
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  // capacity 128, then 256; a security ball past the shell table takes a
+  // table twice as long, up to the full order
+  int L = kTableL, vpl = 2;
+  for (;;) {
+    Table* t = c->table(d->cpd, G.sx, G.sy, G.sz, L, &st);
+    if (!t) return st;
     CellArgs a = base_args(d, t);
+    a.fpos = nullptr;
+    a.arena_next = nullptr;
+    a.summary = nullptr;
     a.work = site ? nullptr : d->work1.p;
     a.ext = site ? d->one_site.p : nullptr;
     a.nwork = 1;
@@ -730,7 +764,7 @@ tess_status single_cell(tess_diagram* d, size_t index, const double* site, doubl
     a.status = status_p;
     a.snbr = snbr;
     a.sarea = sarea;
-    a.stats = site ? nullptr : a.stats;
+    a.stats = nullptr;
     a.exp_v = d->exp_v.p;
     a.exp_ring = d->exp_ring.p;
     a.exp_meta = d->exp_meta.p;
@@ -738,11 +772,16 @@ tess_status single_cell(tess_diagram* d, size_t index, const double* site, doubl
     a.nredo = nullptr;
     if ((!site && (st = c->hip(hipMemcpyAsync(d->work1.p, d->inv.p + oi, 4, hipMemcpyDeviceToDevice, s), "copy")) !=
                       TESS_OK) ||
-        (st = c->hip(launch_cells((int)tries[attempt], a, s), "cell kernel")) != TESS_OK ||
-        (st = c->hip(hipMemcpyAsync(&status, status_p + oi, 4, hipMemcpyDeviceToHost, s), "copy")) != TESS_OK ||
+        (st = c->hip(launch_cells(vpl, a, s), "cell kernel")) != TESS_OK ||
+        (st = c->hip(hipMemcpyAsync(&status, d->one_u32.p + 1, 4, hipMemcpyDeviceToHost, s), "copy")) != TESS_OK ||
         (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
       return st;
-    if (!(status & TESS_CELL_CAPACITY)) break;
+    if ((status & TESS_CELL_CAPACITY) && vpl < 4)
+      vpl = 4;
+    else if ((status & TESS_CELL_TABLE) && L < d->cpd - 1)
+      L = (int)std::min<int64_t>(2 * (int64_t)L, d->cpd - 1);
+    else
+      break;
   }
   if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_TABLE))
     return c->fail(status & TESS_CELL_DEGENERATE ? TESS_E_DEGENERATE : TESS_E_CAPACITY,
@@ -751,10 +790,10 @@ tess_status single_cell(tess_diagram* d, size_t index, const double* site, doubl
   uint32_t fc = 0;
   std::vector<int32_t> nb(kFaceStage), ring(kFaceStage * 64), meta(2 + 256);
   std::vector<double> ar(kFaceStage), vv(3 * 256);
-  hipError_t e = hipMemcpy(&cell->volume, vol + oi, 8, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(&fc, fcount + oi, 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(nb.data(), snbr + (size_t)oi * kFaceStage, 4 * kFaceStage, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(ar.data(), sarea + (size_t)oi * kFaceStage, 8 * kFaceStage, hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpy(&cell->volume, d->one_vol.p, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&fc, d->one_u32.p, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nb.data(), d->one_nbr.p, 4 * kFaceStage, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ar.data(), d->one_area.p, 8 * kFaceStage, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(ring.data(), d->exp_ring.p, 4 * ring.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(meta.data(), d->exp_meta.p, 4 * meta.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(vv.data(), d->exp_v.p, 8 * vv.size(), hipMemcpyDeviceToHost);

@@ -14,14 +14,18 @@
 //   graph 2 (captured per schedule shape, then replayed): the first pass over
 //            the item list (one wavefront per cell, dispatched in list order),
 //            a persistent rerun pass at the largest capacity for the cells the
-//            first pass could not hold, and the face-offset scan
-//   host:    total faces, then CSR compaction of all sets at once
+//            first pass could not hold
+//   host:    cells past the shell table or the largest LDS capacity, if
+//            any, finished set by set (finish_cells); the face-offset scan,
+//            total faces, then CSR compaction of all sets at once
+// With config.eager the same passes are launched directly instead.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <new>
 #include <sstream>
 #include <thread>
@@ -52,10 +56,16 @@ struct Graph {
 
 }  // namespace
 
+// Host words the device results are read back into (pinned).
+struct Mail {
+  uint32_t bad, nredo, ntab, nspill;
+  uint64_t total, arena;
+};
+
 struct tess_batch {
   tess_ctx* ctx = nullptr;
   tess_opts opts;
-  tess_batch_config cfg{64, 0, 4, 0};  // capacity 64, interleaved, chunk 4, list order
+  tess_batch_config cfg{64, 0, 4, 0, 0};  // capacity 64, interleaved, chunk 4, list order, graphs
   size_t nsets = 0, N = 0;
   std::vector<size_t> n, pbase;                        // points per set, first point
   std::vector<size_t> bbase, cbase, sbase, partbase;  // delim, bins, scan sums, partial bounds
@@ -65,6 +75,16 @@ struct tess_batch {
   DevBuf<uint32_t> bins, cnt, delim, cursor, sorted, scratch, sums, big, misc, fcount, status, ctrl;
   DevBuf<int32_t> snbr, nbr;
   DevBuf<uint64_t> foff, fsums, items, redo, base_dev, dend_dev;
+  // face arena of all sets, shell-table exhaustions, cells for the spill kernel
+  DevBuf<uint64_t> fpos, tredo, spill;
+  DevBuf<unsigned long long> arena;
+  Mail* mail = nullptr;       // pinned
+  GridDev* hgrid_pin = nullptr;  // pinned, nsets
+  uint64_t table_reruns = 0, spill_cells = 0;
+  ~tess_batch() {
+    if (mail) (void)hipHostFree(mail);
+    if (hgrid_pin) (void)hipHostFree(hgrid_pin);
+  }
   DevBuf<tess_cell_stats> stats;
   DevBuf<Summary> summary;
   DevBuf<CellArgs> args;
@@ -88,8 +108,10 @@ struct tess_batch {
 namespace {
 
 // ctrl words: [0] rerun count, [1] rerun counter, [2] binning check, [3..6]
-// slot bounds of the learned order
-constexpr int kCtrlRedo = 0, kCtrlQueue = 1, kCtrlBad = 2, kCtrlBounds = 3, kCtrlWords = 8;
+// slot bounds of the learned order, [8] shell-table exhaustions, [9] cells
+// for the spill kernel
+constexpr int kCtrlRedo = 0, kCtrlQueue = 1, kCtrlBad = 2, kCtrlBounds = 3, kCtrlTab = 8, kCtrlSpill = 9,
+              kCtrlWords = 12;
 
 tess_status make_views(tess_batch* b) {
   tess_ctx* c = b->ctx;
@@ -121,8 +143,12 @@ tess_status make_views(tess_batch* b) {
     d->volume.view(b->volume.p + p, d->n);
     d->fcount.view(b->fcount.p + p, d->n);
     d->status.view(b->status.p + p, d->n);
-    d->snbr.view(b->snbr.p + p * kFaceStage, d->n * kFaceStage);
-    d->sarea.view(b->sarea.p + p * kFaceStage, d->n * kFaceStage);
+    // every set writes into the batch's face arena
+    d->snbr.view(b->snbr.p, b->N * kFaceStage);
+    d->sarea.view(b->sarea.p, b->N * kFaceStage);
+    d->fpos.view(b->fpos.p + p, d->n);
+    d->arena_next = b->arena.p;
+    d->arena_cap = (uint64_t)b->N * kFaceStage;
     d->summary.view(b->summary.p + s, 1);
     if (b->opts.collect_stats) d->stats.view(b->stats.p + p, d->n);
     for (int k = 0; k < 3; ++k) {
@@ -229,6 +255,7 @@ tess_status prepare_tables(tess_batch* b, double* table_ms) {
       t.kcut = h.kcut;
       t.L = kTableL;
       ha[s] = base_args(b->views[s].get(), &t);
+      ha[s].defer_table = 1;  // the passes list them (BatchArgs::tredo)
     }
     tess_status st;
     if ((st = c->hip(hipMemcpy(b->tkey.p, hk.data(), 8 * total, hipMemcpyHostToDevice), "upload")) != TESS_OK ||
@@ -283,34 +310,41 @@ tess_status learned_order(tess_batch* b, const tess_model* m, uint32_t bounds[4]
 
 int vpl_of(int capacity) { return capacity >= 256 ? 4 : (capacity >= 128 ? 2 : 1); }
 
-// First pass(es), rerun pass and face-offset scan.
+// First pass(es) and the rerun pass.
 tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32_t bounds[4], bool learned) {
   tess_ctx* c = b->ctx;
   hipStream_t s = c->stream;
   tess_status st;
-  if (launch_fill_u32(b->ctrl.p, 2, 0u, s) != hipSuccess ||
+  if (launch_fill_u32(b->ctrl.p, 2, 0u, s) != hipSuccess || launch_fill_u32(b->ctrl.p + kCtrlTab, 2, 0u, s) != hipSuccess ||
+      launch_fill_u32((uint32_t*)b->arena.p, 2, 0u, s) != hipSuccess ||
       launch_fill_u32((uint32_t*)b->summary.p, b->nsets * (sizeof(Summary) / 4), 0u, s) != hipSuccess)
     return c->fail(TESS_E_DEVICE, "fill");
   BatchArgs a;
   std::memset(&a, 0, sizeof(a));
   a.sets = b->args.p;
   a.items = b->items.p;
-  a.redo = b->redo.p;
-  a.nredo = b->ctrl.p + kCtrlRedo;
+  a.tredo = b->tredo.p;
+  a.ntredo = b->ctrl.p + kCtrlTab;
+  // capacity overflows below 256 vertices go to the rerun list, at 256 to
+  // the spill kernel's
+  auto overflow_to = [&](BatchArgs& x, int vpl) {
+    x.redo = vpl < 4 ? b->redo.p : b->spill.p;
+    x.nredo = b->ctrl.p + (vpl < 4 ? kCtrlRedo : kCtrlSpill);
+  };
   if (learned) {
     static const int kVpl[3] = {1, 2, 4};
     for (int slot = 2; slot >= 0; --slot) {
       const uint32_t cnt = bounds[slot + 1] - bounds[slot];
       if (!cnt) continue;
       a.order = list_base + bounds[slot];
-      a.redo = slot < 2 ? b->redo.p : nullptr;
+      overflow_to(a, kVpl[slot]);
       a.xcd_chunk = 0;
       if ((st = c->hip(launch_cells_items(kVpl[slot], a, cnt, s), "cell pass")) != TESS_OK) return st;
     }
   } else {
     const int vpl = vpl_of(b->cfg.first_capacity);
     a.order = nullptr;
-    a.redo = vpl < 4 ? b->redo.p : nullptr;
+    overflow_to(a, vpl);
     a.xcd_chunk = b->cfg.xcd_chunk;
     if ((st = c->hip(launch_cells_items(vpl, a, (uint32_t)b->N, s), "cell pass")) != TESS_OK) return st;
   }
@@ -322,11 +356,31 @@ tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32
   r.count = b->ctrl.p + kCtrlRedo;
   r.counter = b->ctrl.p + kCtrlQueue;
   r.chunk = std::max(1, b->cfg.redo_chunk);
+  overflow_to(r, 4);
+  r.tredo = b->tredo.p;
+  r.ntredo = b->ctrl.p + kCtrlTab;
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0)
     ncu = 256;
-  if ((st = c->hip(launch_cells_queue(r, ncu * cells_queue_blocks_per_cu(), s), "rerun pass")) != TESS_OK) return st;
-  return c->hip(launch_face_offsets(b->fcount.p, (int)b->N, b->foff.p, b->fsums.p, s), "face offsets");
+  return c->hip(launch_cells_queue(r, ncu * cells_queue_blocks_per_cu(), s), "rerun pass");
+}
+
+// Cells of one set that the batched passes could not finish (shell table,
+// capacity 256), finished like a single diagram's.
+tess_status finish_set(tess_batch* b, size_t set, std::vector<uint32_t>& tab, std::vector<uint32_t>& spl,
+                       uint64_t* failed) {
+  tess_ctx* c = b->ctx;
+  tess_diagram* d = b->views[set].get();
+  tess_status st;
+  Table* t = c->table(d->cpd, d->hgrid.sx, d->hgrid.sy, d->hgrid.sz, kTableL, &st);
+  if (!t) return st;
+  CellArgs a = base_args(d, t);
+  FinishCounts fc;
+  if ((st = finish_cells(d, a, std::move(tab), std::move(spl), &fc)) != TESS_OK) return st;
+  b->table_reruns += fc.table;
+  b->spill_cells += fc.spill;
+  *failed += fc.failed;
+  return TESS_OK;
 }
 
 tess_status batch_compute(tess_batch* b, const tess_model* m) {
@@ -342,50 +396,99 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
       for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
     }
   } guard{ev};
-  if (!b->gbin.exec && (st = build_bin_graph(b)) != TESS_OK) return st;
-  (void)hipEventRecord(ev[0], s);
-  if ((st = c->hip(hipGraphLaunch(b->gbin.exec, s), "binning graph")) != TESS_OK) return st;
+  const bool eager = b->cfg.eager != 0;
+  Mail* mail = b->mail;
+  if (eager) {
+    // the binning passes of every set, launched directly
+    if ((st = c->hip(hipEventRecord(ev[0], s), "event")) != TESS_OK ||
+        (st = c->hip(launch_fill_u32(b->ctrl.p + kCtrlBad, 1, 0u, s), "fill")) != TESS_OK)
+      return st;
+    for (size_t k = 0; k < b->nsets; ++k)
+      if ((st = bin_enqueue(b->views[k].get())) != TESS_OK) return st;
+    if ((st = c->hip(launch_batch_check(b->delim.p, b->dend_dev.p, b->base_dev.p, (int)b->nsets, b->ctrl.p + kCtrlBad,
+                                        s),
+                     "binning check")) != TESS_OK)
+      return st;
+  } else {
+    if (!b->gbin.exec && (st = build_bin_graph(b)) != TESS_OK) return st;
+    (void)hipEventRecord(ev[0], s);
+    if ((st = c->hip(hipGraphLaunch(b->gbin.exec, s), "binning graph")) != TESS_OK) return st;
+  }
   (void)hipEventRecord(ev[1], s);
-  uint32_t bad = 0;
-  if ((st = c->hip(hipMemcpyAsync(b->hgrid.data(), b->grid.p, sizeof(GridDev) * b->nsets, hipMemcpyDeviceToHost, s),
+  // read-backs go to pinned host memory
+  if ((st = c->hip(hipMemcpyAsync(b->hgrid_pin, b->grid.p, sizeof(GridDev) * b->nsets, hipMemcpyDeviceToHost, s),
                    "grid copy")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&bad, b->ctrl.p + kCtrlBad, 4, hipMemcpyDeviceToHost, s), "grid copy")) !=
+      (st = c->hip(hipMemcpyAsync(&mail->bad, b->ctrl.p + kCtrlBad, 4, hipMemcpyDeviceToHost, s), "grid copy")) !=
           TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "binning")) != TESS_OK)
     return st;
-  if (bad) return c->fail(TESS_E_DEVICE, std::to_string(bad) + " sets lost points in the counting sort");
-  for (size_t k = 0; k < b->nsets; ++k) b->views[k]->hgrid = b->hgrid[k];
+  if (mail->bad) return c->fail(TESS_E_DEVICE, std::to_string(mail->bad) + " sets lost points in the counting sort");
+  for (size_t k = 0; k < b->nsets; ++k) {
+    b->hgrid[k] = b->hgrid_pin[k];
+    b->views[k]->hgrid = b->hgrid[k];
+  }
   double table_ms = 0;
   if ((st = prepare_tables(b, &table_ms)) != TESS_OK) return st;
   const bool learned = b->cfg.learned != 0;
   uint32_t bounds[4] = {0, (uint32_t)b->N, (uint32_t)b->N, (uint32_t)b->N};
   if (learned && (st = learned_order(b, m, bounds)) != TESS_OK) return st;
   (void)hipEventRecord(ev[2], s);
-  std::vector<int64_t> key = {b->cfg.first_capacity, b->cfg.xcd_chunk, b->cfg.redo_chunk, b->cfg.learned,
-                              bounds[1], bounds[2]};
-  if (!b->gcells.exec || key != b->gcells_key) {
-    if ((st = capture_begin(c)) != TESS_OK) return st;
-    tess_status err = enqueue_cells(b, b->order.p, bounds, learned);
-    st = capture_end(c, b->gcells);
-    if (err != TESS_OK) return err;
-    if (st != TESS_OK) return st;
-    b->gcells_key = key;
+  if (eager) {
+    if ((st = enqueue_cells(b, b->order.p, bounds, learned)) != TESS_OK) return st;
+  } else {
+    std::vector<int64_t> key = {b->cfg.first_capacity, b->cfg.xcd_chunk, b->cfg.redo_chunk, b->cfg.learned,
+                                bounds[1], bounds[2]};
+    if (!b->gcells.exec || key != b->gcells_key) {
+      if ((st = capture_begin(c)) != TESS_OK) return st;
+      tess_status err = enqueue_cells(b, b->order.p, bounds, learned);
+      st = capture_end(c, b->gcells);
+      if (err != TESS_OK) return err;
+      if (st != TESS_OK) return st;
+      b->gcells_key = key;
+    }
+    if ((st = c->hip(hipGraphLaunch(b->gcells.exec, s), "cells graph")) != TESS_OK) return st;
   }
-  if ((st = c->hip(hipGraphLaunch(b->gcells.exec, s), "cells graph")) != TESS_OK) return st;
-  (void)hipEventRecord(ev[3], s);
-  uint64_t total = 0;
-  uint32_t nredo = 0;
   std::vector<Summary> sm(b->nsets);
-  if ((st = c->hip(hipMemcpyAsync(&total, b->foff.p + b->N, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&nredo, b->ctrl.p + kCtrlRedo, 4, hipMemcpyDeviceToHost, s), "redo")) !=
+  if ((st = c->hip(hipMemcpyAsync(&mail->nredo, b->ctrl.p + kCtrlRedo, 4, hipMemcpyDeviceToHost, s), "redo")) !=
           TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&mail->ntab, b->ctrl.p + kCtrlTab, 8, hipMemcpyDeviceToHost, s), "lists")) !=
+          TESS_OK ||
+      (st = c->hip(hipStreamSynchronize(s), "cells")) != TESS_OK)
+    return st;
+  const uint32_t nredo = mail->nredo, ntab = mail->ntab, nspill = mail->nspill;
+  // cells past the shell table or capacity 256 (rare): finished set by set
+  uint64_t failed = 0;
+  b->table_reruns = b->spill_cells = 0;
+  if (ntab || nspill) {
+    std::vector<uint64_t> t64(ntab), s64(nspill);
+    if ((ntab && (st = c->hip(hipMemcpy(t64.data(), b->tredo.p, 8ull * ntab, hipMemcpyDeviceToHost), "download")) !=
+                     TESS_OK) ||
+        (nspill &&
+         (st = c->hip(hipMemcpy(s64.data(), b->spill.p, 8ull * nspill, hipMemcpyDeviceToHost), "download")) != TESS_OK))
+      return st;
+    std::map<size_t, std::pair<std::vector<uint32_t>, std::vector<uint32_t>>> per;
+    for (uint64_t it : t64) per[it >> 32].first.push_back((uint32_t)it);
+    for (uint64_t it : s64) per[it >> 32].second.push_back((uint32_t)it);
+    for (auto& kv : per)
+      if ((st = finish_set(b, kv.first, kv.second.first, kv.second.second, &failed)) != TESS_OK) return st;
+  }
+  if ((st = c->hip(launch_face_offsets(b->fcount.p, (int)b->N, b->foff.p, b->fsums.p, s), "face offsets")) !=
+      TESS_OK)
+    return st;
+  (void)hipEventRecord(ev[3], s);
+  if ((st = c->hip(hipMemcpyAsync(&mail->total, b->foff.p + b->N, 8, hipMemcpyDeviceToHost, s), "total")) !=
+          TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&mail->arena, b->arena.p, 8, hipMemcpyDeviceToHost, s), "arena")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(sm.data(), b->summary.p, sizeof(Summary) * b->nsets, hipMemcpyDeviceToHost, s),
                    "summary")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "cells")) != TESS_OK)
     return st;
+  const uint64_t total = mail->total;
+  if (mail->arena > (uint64_t)b->N * kFaceStage) return c->fail(TESS_E_CAPACITY, "the face arena is full");
   if (b->nbr.alloc(total) != hipSuccess || b->area.alloc(total) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
-  if ((st = c->hip(launch_compact(b->fcount.p, b->foff.p, (int)b->N, b->snbr.p, b->sarea.p, b->nbr.p, b->area.p, s),
+  if ((st = c->hip(launch_compact(b->fcount.p, b->foff.p, (int)b->N, b->snbr.p, b->sarea.p, b->nbr.p, b->area.p, s,
+                                  b->fpos.p),
                    "compact")) != TESS_OK)
     return st;
   (void)hipEventRecord(ev[4], s);
@@ -405,19 +508,19 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
   b->tim.table_ms = table_ms;
   b->tim.capacity_reruns = nredo;
   b->tim.fallback_cells = learned ? nredo : 0;
-  uint64_t cap = 0, tab = 0, bad_cells = 0, flops = 0;
+  b->tim.table_reruns = b->table_reruns;
+  b->tim.spill_cells = b->spill_cells;
+  uint64_t bad_cells = 0, flops = 0, inc = 0;
   for (const Summary& x : sm) {
-    cap += x.capacity;
-    tab += x.table;
     bad_cells += x.degenerate;
     flops += x.flops;
+    inc += x.incident_cuts;
   }
-  cap -= nredo;  // first-pass overflows were rerun; what overflowed again failed
+  b->tim.incident_cuts = inc;
   b->tim.flops = flops;
   b->total_faces = total;
   b->computed = true;
-  if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
-  if (tab) return c->fail(TESS_E_CAPACITY, std::to_string(tab) + " cells exhausted the shell table");
+  if (failed) return c->fail(TESS_E_CAPACITY, std::to_string(failed) + " cells exceeded the spill workspace limit");
   if (bad_cells) return c->fail(TESS_E_DEGENERATE, std::to_string(bad_cells) + " cells failed topology checks");
   return TESS_OK;
 }
@@ -425,7 +528,7 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
 bool valid_config(const tess_batch_config& k) {
   return (k.first_capacity == 64 || k.first_capacity == 128 || k.first_capacity == 256) &&
          k.xcd_chunk >= 0 && k.xcd_chunk <= (1 << 24) && k.redo_chunk >= 1 && k.redo_chunk <= 64 &&
-         (k.learned == 0 || k.learned == 1);
+         (k.learned == 0 || k.learned == 1) && (k.eager == 0 || k.eager == 1);
 }
 
 std::string signature(const tess_batch* b, bool learned) {
@@ -500,7 +603,10 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
             b->items.alloc(N) == hipSuccess && b->redo.alloc(N) == hipSuccess &&
             b->base_dev.alloc(nsets + 1) == hipSuccess && b->dend_dev.alloc(nsets) == hipSuccess &&
             b->summary.alloc(nsets) == hipSuccess && b->args.alloc(nsets) == hipSuccess &&
-            b->ctrl.alloc(kCtrlWords) == hipSuccess;
+            b->ctrl.alloc(kCtrlWords) == hipSuccess && b->fpos.alloc(N) == hipSuccess &&
+            b->tredo.alloc(N) == hipSuccess && b->spill.alloc(N) == hipSuccess && b->arena.alloc(1) == hipSuccess &&
+            hipHostMalloc((void**)&b->mail, sizeof(Mail), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc((void**)&b->hgrid_pin, sizeof(GridDev) * nsets, hipHostMallocDefault) == hipSuccess;
   if (ok && b->opts.collect_stats) ok = b->stats.alloc(N) == hipSuccess;
   if (!ok) return ctx->fail(TESS_E_NOMEM, "device allocation failed");
   // points: one AoS upload, one SoA pass
@@ -589,6 +695,7 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
       std::istringstream is(line);
       std::string s;
       tess_batch_config k;
+      k.eager = 0;
       double ms;
       if (is >> s >> k.first_capacity >> k.xcd_chunk >> k.redo_chunk >> k.learned >> ms && s == sig &&
           valid_config(k) && (!k.learned || can_learn)) {
@@ -600,9 +707,9 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
   }
   std::vector<tess_batch_config> cand;
   for (int cap : {64, 128})
-    for (int xcd : {0, 256, 4096}) cand.push_back({cap, xcd, 4, 0});
+    for (int xcd : {0, 256, 4096}) cand.push_back({cap, xcd, 4, 0, 0});
   if (can_learn)
-    for (int xcd : {0, 256}) cand.push_back({64, xcd, 4, 1});
+    for (int xcd : {0, 256}) cand.push_back({64, xcd, 4, 1, 0});
   double best_ms = 1e300;
   tess_batch_config bk = b->cfg;
   for (const tess_batch_config& k : cand) {

@@ -369,15 +369,17 @@ __global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ fc
                                                  const uint64_t* __restrict__ foff, int n,
                                                  const int32_t* __restrict__ snbr,
                                                  const double* __restrict__ sarea,
-                                                 int32_t* __restrict__ nbr, double* __restrict__ area) {
+                                                 int32_t* __restrict__ nbr, double* __restrict__ area,
+                                                 const uint64_t* __restrict__ fpos) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int cell = g >> 4, sub = g & 15;
   if (cell >= n) return;
   const uint32_t fc = fcount[cell];
   const uint64_t o = foff[cell];
+  const uint64_t src = fpos ? (fc ? fpos[cell] : 0) : (uint64_t)cell * kFaceStage;
   for (uint32_t f = sub; f < fc; f += 16) {
-    nbr[o + f] = snbr[(size_t)cell * kFaceStage + f];
-    area[o + f] = sarea[(size_t)cell * kFaceStage + f];
+    nbr[o + f] = snbr[src + f];
+    area[o + f] = sarea[src + f];
   }
 }
 
@@ -470,9 +472,9 @@ hipError_t launch_face_offsets(const uint32_t* fcount, int n, uint64_t* foff, ui
 }
 
 hipError_t launch_compact(const uint32_t* fcount, const uint64_t* foff, int n, const int32_t* snbr,
-                          const double* sarea, int32_t* nbr, double* area, hipStream_t s) {
+                          const double* sarea, int32_t* nbr, double* area, hipStream_t s, const uint64_t* fpos) {
   hipLaunchKernelGGL(k_compact, dim3(blocks((long long)n * 16, 256)), dim3(256), 0, s, fcount, foff, n,
-                     snbr, sarea, nbr, area);
+                     snbr, sarea, nbr, area, fpos);
   return hipGetLastError();
 }
 

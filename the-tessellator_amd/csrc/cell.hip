@@ -27,6 +27,8 @@
 // is one IEEE-rounded operation in the same order as the CPU restatement.
 #include <math.h>
 
+#include <type_traits>
+
 #include "launch.h"
 #include "tess_internal.h"
 
@@ -222,6 +224,8 @@ __constant__ int32_t kWallTag[6] = {-3, -2, -4, -1, -6, -5};
 template <int VPL, int PPL, int FPL>
 struct Lds {
   static constexpr int VC = 64 * VPL, PC = 64 * PPL, HC = 2 * PC, FC = 64 * FPL;
+  // a pair slot number: one byte while there are at most 256 pairs
+  using PairSlot = typename std::conditional<(PC > 256), uint16_t, uint8_t>::type;
   double vx[VC], vy[VC], vz[VC], vdot[VC];
   // kept candidates of the current chunk (in stream order); the output
   // phase reuses the space for per-face volume terms
@@ -254,7 +258,7 @@ struct Lds {
     struct {
       uint16_t cpair[64];    // crossing pair k: 2 * pair + (half-edge 2p points Inside)
       uint8_t vslot[64];     // k-th free vertex slot
-      uint8_t pslot[64];     // k-th free pair slot
+      PairSlot pslot[64];    // k-th free pair slot
       uint8_t rank[PC];      // rank of a crossing pair
     } f;
   } cs;
@@ -371,7 +375,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     for (int r = 0; r < PPL; ++r) {
       const int k = bp + mbcnt(pfree[r], lane);
       const bool take = ((pfree[r] >> lane) & 1ull) && k < ncross;
-      if (take) S.cs.f.pslot[k] = (uint8_t)(lane + 64 * r);
+      if (take) S.cs.f.pslot[k] = (typename Lds<VPL, PPL, FPL>::PairSlot)(lane + 64 * r);
       pnew[r] = ballot(take);
       bp += popc(pfree[r]);
     }
@@ -796,7 +800,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   int r2_slot = 0;
   double r2 = cell_r2_slot(C, lane, &r2_slot);
 
-  uint32_t streamed = 0, classified = 0, ncuts = 0, status = 0;
+  uint32_t streamed = 0, classified = 0, ncuts = 0, ninc = 0, status = 0;
   uint64_t flops = 24 + 40;
   int maxv = 8;
   int e_lastcut = -1;
@@ -807,8 +811,10 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 
   for (int ebase = 0; !done; ebase += 64) {
     if (ebase >= A.tlen) {
-      // table exhausted: every later entry has key >= kcut
-      if (!(A.kcut > stop_thr(r2))) status |= TESS_CELL_TABLE;
+      // table exhausted: every later entry has key >= kcut, so the stream
+      // has ended when kcut is past the security threshold or the radius
+      // limit; otherwise the cell needs a longer table
+      if (!(A.kcut > stop_thr(r2)) && !(A.kcut > rlim)) status |= TESS_CELL_TABLE;
       e_stop_final = A.tlen;
       break;
     }
@@ -992,6 +998,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           break;
         }
         ncuts++;
+        ninc += newv == 0;  // no Inside-Outside edge: only Incident vertices border the cut
         nv = popc_multi(C.VA);
         if (nv > maxv) maxv = nv;
         flops += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
@@ -1062,9 +1069,27 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     const int nv = popc_multi(C.VA), ne = popc_multi(C.PA);
     if (nv - ne + nfaces != 2) status |= TESS_CELL_DEGENERATE;
   }
-  if (nfaces > kFaceStage) status |= TESS_CELL_CAPACITY;
+  // per-cell face slots hold kFaceStage faces; the arena has no such limit
+  if (nfaces > kFaceStage && (GEO != 0 || A.fpos == nullptr)) status |= TESS_CELL_CAPACITY;
+  // a cell that reached the end of the table is recomputed with a longer
+  // one when the caller collects them: it writes nothing now
+  const bool table_rerun = (status & TESS_CELL_TABLE) && A.defer_table;
+  // where the faces go: an arena range reserved by one atomic, or the
+  // cell's own slots
+  uint64_t fbase = (uint64_t)oi * kFaceStage;
+  if (GEO != 1 && A.fpos != nullptr && !table_rerun && (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0) {
+    unsigned long long p = 0;
+    if (lane == 0) p = atomicAdd(A.arena_next, (unsigned long long)nfaces);
+    const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p & 0xffffffffull));
+    const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p >> 32));
+    fbase = ((uint64_t)phi << 32) | plo;
+    if (fbase + (uint64_t)nfaces > A.arena_cap)
+      status |= TESS_CELL_CAPACITY;  // arena full (the caller reports it)
+    else if (lane == 0)
+      A.fpos[oi] = fbase;
+  }
   uint64_t oflops = 0;
-  if ((status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0) {
+  if ((status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0 && !table_rerun) {
     double area[FPL], contrib[FPL];
     int32_t tagv[FPL];
     int seqv[FPL];
@@ -1159,8 +1184,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 #pragma unroll
       for (int r = 0; r < FPL; ++r)
         if (fal[r]) {
-          A.snbr[(size_t)oi * kFaceStage + rt[r]] = tagv[r];
-          A.sarea[(size_t)oi * kFaceStage + rt[r]] = area[r];
+          A.snbr[fbase + rt[r]] = tagv[r];
+          A.sarea[fbase + rt[r]] = area[r];
         }
     }
     if (A.exp_v) {
@@ -1260,9 +1285,10 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       A.status[oi] = status;
     }
     if (A.summary) {
-      if (!(status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE))) {
+      if (!(status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) && !table_rerun) {
         atomicAdd(&A.summary->flops, (unsigned long long)(flops + oflops + 1));
         atomicAdd(&A.summary->candidates, (unsigned long long)streamed);
+        if (ninc) atomicAdd(&A.summary->incident_cuts, (unsigned long long)ninc);
       }
       if (status & TESS_CELL_DEGENERATE) atomicAdd(&A.summary->degenerate, 1u);
       if (status & TESS_CELL_CAPACITY) atomicAdd(&A.summary->capacity, 1u);
@@ -1278,6 +1304,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       st.r2 = r2;
       st.max_vertices = (uint32_t)maxv;
       st.status = status;
+      st.incident_cuts = ninc;
+      st.reserved = 0;
       A.stats[oi] = st;
     }
   }
@@ -1298,6 +1326,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t slot = atomicAdd(A.nredo, 1u);
     A.redo[slot] = (uint32_t)gpos;
   }
+  if (lane == 0 && (status & TESS_CELL_TABLE) && A.tredo) {
+    const uint32_t slot = atomicAdd(A.ntredo, 1u);
+    A.tredo[slot] = (uint32_t)gpos;
+  }
 }
 
 // Cells of many sets in one launch.  An item is (set << 32) | grid position;
@@ -1312,6 +1344,10 @@ __device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<
   if (lane == 0 && (status & TESS_CELL_CAPACITY) && B.redo) {
     const uint32_t slot = atomicAdd(B.nredo, 1u);
     B.redo[slot] = it;
+  }
+  if (lane == 0 && (status & TESS_CELL_TABLE) && B.tredo) {
+    const uint32_t slot = atomicAdd(B.ntredo, 1u);
+    B.tredo[slot] = it;
   }
 }
 
@@ -1344,6 +1380,491 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       wave_sync();
       batch_item(B, B.items[B.order ? B.order[i] : i], S, lane);
     }
+  }
+}
+
+// ---------------------------------------------------------------- spill
+// Cells beyond the LDS kernels (more than 256 vertices or 128 faces, e.g. a
+// site ringed by hundreds of points).  One wavefront per cell; the
+// polyhedron is its list of face rings (site-relative vertex coordinates,
+// counter-clockwise seen from outside) in creation order, in a global-memory
+// workspace that the host grows on demand.  A cut classifies every ring
+// entry (vector3.rs:170-193), clips each ring on its own -- an Outside run is
+// replaced by its entry and exit points, a point on an Inside-Outside edge
+// being intersection(outside, inside) (vector3.rs:213-219) and an Incident
+// end kept as is -- and closes the new face from the runs' exit -> entry
+// segments, tagged with the cutting point (polyhedron.rs:438-642 with the
+// walk's defects fixed).  The arithmetic and the face order are those of the
+// LDS kernels, so a cell both can build is bitwise the same.
+
+__device__ __forceinline__ void ws_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ double rl_d(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+struct SpillWs {
+  double* ring[2];    // [rmax][3]
+  double* seg;        // [fmax][6]: exit point, entry point; the output phase
+                      // keeps face volume terms and areas here
+  uint32_t* foff[2];  // [fmax + 1]
+  int32_t* ftag[2];   // [fmax]
+  int32_t* snext;     // [fmax]
+  uint32_t* fterm;    // [fmax][4]: kept length, segments, ring offset, segment offset
+  uint8_t* loc;       // [rmax]
+  uint8_t* used;      // [fmax]
+};
+
+__device__ __forceinline__ void spill_ws(const SpillArgs& P, int blk, SpillWs& W) {
+  double* d = P.wsd + (size_t)blk * spill_wsd_doubles(P.rmax, P.fmax);
+  uint32_t* u = P.wsu + (size_t)blk * spill_wsu_words(P.fmax);
+  uint8_t* b = P.wsb + (size_t)blk * spill_wsb_bytes(P.rmax, P.fmax);
+  W.ring[0] = d;
+  W.ring[1] = d + 3ull * P.rmax;
+  W.seg = d + 6ull * P.rmax;
+  W.foff[0] = u;
+  W.foff[1] = u + (P.fmax + 1);
+  W.ftag[0] = (int32_t*)(u + 2 * (P.fmax + 1));
+  W.ftag[1] = W.ftag[0] + P.fmax;
+  W.snext = W.ftag[1] + P.fmax;
+  W.fterm = (uint32_t*)(W.snext + P.fmax);
+  W.loc = b;
+  W.used = b + P.rmax;
+}
+
+// Clips the ring R[a, a + k) (classes L[a, a + k)).  Counts the kept ring
+// entries (olen), the exit -> entry segments (nseg) and the crossing edges
+// created on their exit side (nnew); writes the kept ring to out and the
+// segments to segs when those are not null.
+__device__ void spill_clip_ring(const double* R, const uint8_t* L, int a, int k, double nx, double ny, double nz,
+                                double off, double* out, double* segs, int& olen, int& nseg, int& nnew) {
+  olen = 0;
+  nseg = 0;
+  nnew = 0;
+  int start = -1, nout = 0;
+  for (int i = 0; i < k; ++i) {
+    if (L[a + i] == C_OUT)
+      ++nout;
+    else if (start < 0)
+      start = i;
+  }
+  if (nout == 0) {
+    if (out)
+      for (int i = 0; i < 3 * k; ++i) out[i] = R[3 * a + i];
+    olen = k;
+    return;
+  }
+  if (start < 0) return;  // the whole face is cut off
+  auto emit = [&](double x, double y, double z) {
+    if (out) {
+      out[3 * olen] = x;
+      out[3 * olen + 1] = y;
+      out[3 * olen + 2] = z;
+    }
+    ++olen;
+  };
+  // intersection(outside o, inside q), vector3.rs:213-219
+  auto cross_pt = [&](const double* o, const double* q, double& x, double& y, double& z) {
+    const double dO = (nx * o[0] + ny * o[1]) + nz * o[2];
+    const double dQ = (nx * q[0] + ny * q[1]) + nz * q[2];
+    const double t = (off - dO) / (dQ - dO);
+    x = o[0] + (q[0] - o[0]) * t;
+    y = o[1] + (q[1] - o[1]) * t;
+    z = o[2] + (q[2] - o[2]) * t;
+  };
+  double ex = 0.0, ey = 0.0, ez = 0.0;  // entry point of the current Outside run
+  int i = start;
+  for (int t = 0; t < k; ++t) {
+    int j = i + 1;
+    if (j == k) j = 0;
+    const uint8_t ci = L[a + i], cj = L[a + j];
+    const double* vi = R + 3 * (a + i);
+    const double* vj = R + 3 * (a + j);
+    if (ci != C_OUT) {
+      emit(vi[0], vi[1], vi[2]);
+      if (cj == C_OUT) {
+        if (ci == C_IN) {
+          cross_pt(vj, vi, ex, ey, ez);
+          emit(ex, ey, ez);
+        } else {
+          ex = vi[0];
+          ey = vi[1];
+          ez = vi[2];
+        }
+      }
+    } else if (cj != C_OUT) {
+      double x = vj[0], y = vj[1], z = vj[2];
+      if (cj == C_IN) {
+        cross_pt(vi, vj, x, y, z);
+        emit(x, y, z);
+        ++nnew;
+      }
+      if (!(x == ex && y == ey && z == ez)) {
+        if (segs) {
+          double* g = segs + 6 * nseg;
+          g[0] = x;
+          g[1] = y;
+          g[2] = z;
+          g[3] = ex;
+          g[4] = ey;
+          g[5] = ez;
+        }
+        ++nseg;
+      }
+    }
+    i = j;
+  }
+}
+
+enum { SPILL_NOCUT = -1, SPILL_CAPACITY = -2, SPILL_DEGENERATE = -3 };
+
+// One bisector against the soup in buffer b (nf faces, ne2 ring entries).
+// Returns the new vertices (>= 0) when the cell changed -- the soup is then
+// in buffer b ^ 1 with nf, ne2 updated -- or SPILL_*.
+__device__ int spill_cut(SpillWs& W, int b, int& nf, int& ne2, int rmax, int fmax, double nx, double ny, double nz,
+                         double off, int32_t tag, int lane) {
+  const double* R = W.ring[b];
+  bool anyo = false;
+  for (int i = lane; i < ne2; i += 64) {
+    const double sd = ((nx * R[3 * i] + ny * R[3 * i + 1]) + nz * R[3 * i + 2]) - off;
+    const uint8_t c = sd > kTol ? C_OUT : (sd < -kTol ? C_IN : C_INC);
+    W.loc[i] = c;
+    anyo = anyo || c == C_OUT;
+  }
+  if (!ballot(anyo)) return SPILL_NOCUT;
+  ws_sync();
+  const uint32_t* FO = W.foff[b];
+  // pass 1: per face the kept length (0 when fewer than 3 entries remain),
+  // its segments and new vertices; exclusive scans give the offsets
+  int tlen = 0, tseg = 0, tkept = 0, tnew = 0;
+  for (int f0 = 0; f0 < nf; f0 += 64) {
+    const int f = f0 + lane;
+    int olen = 0, nseg = 0, nnew = 0;
+    if (f < nf) {
+      const int a = (int)FO[f], k = (int)(FO[f + 1] - FO[f]);
+      spill_clip_ring(R, W.loc, a, k, nx, ny, nz, off, nullptr, nullptr, olen, nseg, nnew);
+      if (olen < 3) olen = 0;
+    }
+    const int il = wave_incl_scan(olen, lane), is = wave_incl_scan(nseg, lane);
+    const int ik = wave_incl_scan(olen > 0 ? 1 : 0, lane);
+    if (f < nf) {
+      W.fterm[4 * f] = (uint32_t)olen;
+      W.fterm[4 * f + 1] = (uint32_t)(tkept + ik - (olen > 0 ? 1 : 0));
+      W.fterm[4 * f + 2] = (uint32_t)(tlen + il - olen);
+      W.fterm[4 * f + 3] = (uint32_t)(tseg + is - nseg);
+    }
+    tlen += rl_i(il, 63);
+    tseg += rl_i(is, 63);
+    tkept += rl_i(ik, 63);
+    tnew += rl_i(wave_incl_scan(nnew, lane), 63);
+  }
+  const int m = tseg;
+  if (tlen + m > rmax || tkept + 1 > fmax || m > fmax) return SPILL_CAPACITY;
+  if (m < 3) return SPILL_DEGENERATE;
+  ws_sync();
+  // pass 2: kept rings and segments into place
+  double* RN = W.ring[b ^ 1];
+  uint32_t* FN = W.foff[b ^ 1];
+  int32_t* TN = W.ftag[b ^ 1];
+  for (int f = lane; f < nf; f += 64) {
+    const int a = (int)FO[f], k = (int)(FO[f + 1] - FO[f]);
+    const int olen = (int)W.fterm[4 * f], kidx = (int)W.fterm[4 * f + 1];
+    const int roff = (int)W.fterm[4 * f + 2], soff = (int)W.fterm[4 * f + 3];
+    int ol, ns, nn;
+    spill_clip_ring(R, W.loc, a, k, nx, ny, nz, off, olen > 0 ? RN + 3 * roff : nullptr, W.seg + 6 * soff, ol, ns,
+                    nn);
+    if (olen > 0) {
+      FN[kidx] = (uint32_t)roff;
+      TN[kidx] = W.ftag[b][f];
+    }
+  }
+  ws_sync();
+  // the new face: segments chained head (exit point) to tail (entry point)
+  for (int g = lane; g < m; g += 64) {
+    const double e0 = W.seg[6 * g + 3], e1 = W.seg[6 * g + 4], e2 = W.seg[6 * g + 5];
+    int nxt = -1;
+    for (int h = 0; h < m; ++h)
+      if (W.seg[6 * h] == e0 && W.seg[6 * h + 1] == e1 && W.seg[6 * h + 2] == e2) {
+        nxt = h;
+        break;
+      }
+    W.snext[g] = nxt;
+    W.used[g] = 0;
+  }
+  ws_sync();
+  int err = 0;
+  if (lane == 0) {
+    int cur = 0, pos = tlen;
+    for (int c = 0; c < m; ++c) {
+      W.used[cur] = 1;
+      RN[3 * pos] = W.seg[6 * cur];
+      RN[3 * pos + 1] = W.seg[6 * cur + 1];
+      RN[3 * pos + 2] = W.seg[6 * cur + 2];
+      ++pos;
+      const int nxt = W.snext[cur];
+      if (nxt < 0) {
+        err = 1;
+        break;
+      }
+      if (nxt == 0) {
+        if (c != m - 1) err = 1;
+        break;
+      }
+      if (W.used[nxt]) {
+        err = 1;
+        break;
+      }
+      cur = nxt;
+    }
+    FN[tkept] = (uint32_t)tlen;
+    TN[tkept] = tag;
+    FN[tkept + 1] = (uint32_t)(tlen + m);
+  }
+  err = uni_i(rl_i(err, 0));
+  ws_sync();
+  if (err) return SPILL_DEGENERATE;
+  nf = tkept + 1;
+  ne2 = tlen + m;
+  return tnew;
+}
+
+__device__ double spill_r2(const double* R, int ne2, int lane) {
+  double m = 0.0;
+  for (int i = lane; i < ne2; i += 64) {
+    const double d = (R[3 * i] * R[3 * i] + R[3 * i + 1] * R[3 * i + 1]) + R[3 * i + 2] * R[3 * i + 2];
+    m = d > m ? d : m;
+  }
+  return wave_max_uni(m);
+}
+
+// The cell of work item gpos of set A (as cell_body) in the spill workspace.
+__device__ void spill_cell(const CellArgs& A, const int gpos, const SpillArgs& P, SpillWs& W, uint64_t item,
+                           int lane) {
+  const bool ext = A.ext != nullptr;
+  const uint32_t self = ext ? 0xffffffffu : A.sorted[gpos];
+  const double sx = ext ? A.ext[3 * gpos] : A.spx[gpos];
+  const double sy = ext ? A.ext[3 * gpos + 1] : A.spy[gpos];
+  const double sz = ext ? A.ext[3 * gpos + 2] : A.spz[gpos];
+  const GridDev G = *A.grid;
+  const int cpd = G.cpd;
+  const int ix = axis_index(sx, G.xmin, G.xmax, G.ix, cpd);
+  const int iy = axis_index(sy, G.ymin, G.ymax, G.iy, cpd);
+  const int iz = axis_index(sz, G.zmin, G.zmax, G.iz, cpd);
+  // container cube translated by -site: face f's ring from its start half-edge
+  {
+    const double x0 = A.lo[0] + (-sx), x1 = A.hi[0] + (-sx);
+    const double y0 = A.lo[1] + (-sy), y1 = A.hi[1] + (-sy);
+    const double z0 = A.lo[2] + (-sz), z1 = A.hi[2] + (-sz);
+    if (lane < 6) {
+      int h = kCubeStart[lane];
+      for (int k = 0; k < 4; ++k) {
+        const int v = kCubeTgt[h];
+        double* o = W.ring[0] + 3 * (4 * lane + k);
+        o[0] = (v == 1 || v == 2 || v == 5 || v == 6) ? x1 : x0;
+        o[1] = (v >= 4) ? y1 : y0;
+        o[2] = (v == 2 || v == 3 || v == 6 || v == 7) ? z1 : z0;
+        h = kCubeNext[h];
+      }
+      W.foff[0][lane] = (uint32_t)(4 * lane);
+      W.ftag[0][lane] = kWallTag[lane];
+    }
+    if (lane == 0) W.foff[0][6] = 24;
+  }
+  ws_sync();
+  int b = 0, nf = 6, ne2 = 24;
+  double r2 = spill_r2(W.ring[0], ne2, lane);
+  uint32_t streamed = 0, classified = 0, ncuts = 0, ninc = 0, status = TESS_CELL_SPILL;
+  uint64_t flops = 24 + 40;
+  int maxv = 8;
+  bool done = false;
+  const double rlim = A.radius_limit;
+  int e = 0;
+  for (; e < A.tlen; ++e) {
+    const double key = A.tkey[e];
+    if (key > stop_thr(r2) || key > rlim) break;
+    int ox, oy, oz;
+    unpack_off(A.toff[e], ox, oy, oz);
+    const int bx = ix + ox, by = iy + oy, bz = iz + oz;
+    if (bx < 0 || bx >= cpd || by < 0 || by >= cpd || bz < 0 || bz >= cpd) continue;
+    const int bin = (bx * cpd + by) * cpd + bz;
+    const uint32_t bst = A.delim[bin];
+    const int cnt = (int)(A.delim[bin + 1] - bst);
+    for (int k0 = 0; k0 < cnt && !done; k0 += 64) {
+      const int k = k0 + lane;
+      bool valid = false;
+      uint32_t jidx = 0;
+      double px = 0, py = 0, pz = 0, d2 = 0;
+      if (k < cnt) {
+        const uint32_t s = bst + (uint32_t)k;
+        jidx = A.sorted[s];
+        valid = (jidx != self) && (A.target_group < 0 || A.groups[jidx] == A.target_group);
+        px = A.spx[s] - sx;
+        py = A.spy[s] - sy;
+        pz = A.spz[s] - sz;
+        d2 = (px * px + py * py) + pz * pz;
+      }
+      double nx = 0, ny = 0, nz = 0, off = 0;
+      if (valid) {  // halfway_from_origin_to, vector3.rs:223-239
+        const double mag = sqrt(d2);
+        const double inv = 1.0 / mag;
+        nx = px * inv;
+        ny = py * inv;
+        nz = pz * inv;
+        off = (nx * (px * 0.5) + ny * (py * 0.5)) + nz * (pz * 0.5);
+      }
+      uint64_t pend = ballot(valid);
+      streamed += (uint32_t)popc(pend);
+      while (pend) {
+        const int q = ctz64(pend);
+        pend &= pend - 1ull;
+        if (rl_d(d2, q) > stop_thr(r2)) continue;
+        const int nv0 = ne2 / 2 - nf + 2;
+        classified++;
+        flops += 13 + 6ull * (uint64_t)nv0;
+        const int rc = spill_cut(W, b, nf, ne2, P.rmax, P.fmax, rl_d(nx, q), rl_d(ny, q), rl_d(nz, q),
+                                 rl_d(off, q), (int32_t)rl_i((int)jidx, q), lane);
+        if (rc == SPILL_NOCUT) continue;
+        if (rc < 0) {
+          status |= rc == SPILL_CAPACITY ? TESS_CELL_CAPACITY : TESS_CELL_DEGENERATE;
+          done = true;
+          break;
+        }
+        b ^= 1;
+        ncuts++;
+        ninc += rc == 0;
+        const int nv = ne2 / 2 - nf + 2;
+        if (nv > maxv) maxv = nv;
+        flops += 12ull * (uint64_t)rc + 5ull * (uint64_t)nv;
+        r2 = spill_r2(W.ring[b], ne2, lane);
+      }
+    }
+    if (done) break;
+  }
+  if (!done && e >= A.tlen && !(A.kcut > stop_thr(r2)) && !(A.kcut > rlim)) status |= TESS_CELL_TABLE;
+  flops += 8ull * streamed;
+
+  const uint32_t oi = ext ? (uint32_t)gpos : self;
+  const bool fail = (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_TABLE)) != 0;
+  uint64_t fbase = (uint64_t)oi * kFaceStage;
+  uint64_t oflops = 0;
+  bool wrote = false;
+  if (!fail) {
+    if (A.fpos != nullptr) {
+      unsigned long long p = 0;
+      if (lane == 0) p = atomicAdd(A.arena_next, (unsigned long long)nf);
+      const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p & 0xffffffffull));
+      const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p >> 32));
+      fbase = ((uint64_t)phi << 32) | plo;
+      if (fbase + (uint64_t)nf > A.arena_cap) status |= TESS_CELL_CAPACITY;
+    } else if (nf > kFaceStage) {
+      status |= TESS_CELL_CAPACITY;
+    }
+  }
+  if (!fail && !(status & TESS_CELL_CAPACITY)) {
+    // per face: the fan from the lexicographically smallest vertex
+    // (polyhedron.rs:776-808), its volume term and area (interface.rs:408-410)
+    const double* R = W.ring[b];
+    const uint32_t* FO = W.foff[b];
+    const int32_t* FT = W.ftag[b];
+    double* contrib = W.seg;
+    double* area = W.seg + P.fmax;
+    for (int f = lane; f < nf; f += 64) {
+      const int a = (int)FO[f], k = (int)(FO[f + 1] - FO[f]);
+      int an = 0;
+      for (int i = 1; i < k; ++i)
+        if (lex_less(R[3 * (a + i)], R[3 * (a + i) + 1], R[3 * (a + i) + 2], R[3 * (a + an)], R[3 * (a + an) + 1],
+                     R[3 * (a + an) + 2]))
+          an = i;
+      const double ax = R[3 * (a + an)], ay = R[3 * (a + an) + 1], az = R[3 * (a + an) + 2];
+      int j1 = an + 1;
+      if (j1 == k) j1 = 0;
+      double cx = R[3 * (a + j1)] - ax, cy = R[3 * (a + j1) + 1] - ay, cz = R[3 * (a + j1) + 2] - az;
+      double wx = 0.0, wy = 0.0, wz = 0.0;
+      int jj = j1;
+      for (int j = 2; j < k; ++j) {
+        ++jj;
+        if (jj == k) jj = 0;
+        const double qx = cx, qy = cy, qz = cz;
+        cx = R[3 * (a + jj)] - ax;
+        cy = R[3 * (a + jj) + 1] - ay;
+        cz = R[3 * (a + jj) + 2] - az;
+        wx = wx + (qy * cz - qz * cy);
+        wy = wy + (qz * cx - qx * cz);
+        wz = wz + (qx * cy - qy * cx);
+      }
+      contrib[f] = (ax * wx + ay * wy) + az * wz;
+      area[f] = 0.5 * sqrt((wx * wx + wy * wy) + wz * wz);
+      oflops += 3ull * (k - 1) + 12ull * (k - 2) + 13;
+    }
+    ws_sync();
+    // neighbours ascending: a face's rank is the number of smaller tags
+    for (int f = lane; f < nf; f += 64) {
+      const int32_t t = FT[f];
+      int rank = 0;
+      for (int g = 0; g < nf; ++g) rank += FT[g] < t;
+      A.snbr[fbase + rank] = t;
+      A.sarea[fbase + rank] = area[f];
+    }
+    if (lane == 0) {
+      double vol = 0.0;
+      for (int f = 0; f < nf; ++f) vol = vol + contrib[f];
+      A.volume[oi] = vol / 6.0;
+      A.fcount[oi] = (uint32_t)nf;
+      if (A.fpos) A.fpos[oi] = fbase;
+    }
+    wrote = true;
+  }
+  oflops = wave_sum_u64(oflops);
+  if (lane == 0) {
+    if (!wrote) {
+      A.volume[oi] = 0.0;
+      A.fcount[oi] = 0;
+    }
+    A.status[oi] = status;
+    if (A.summary) {
+      if (wrote) {
+        atomicAdd(&A.summary->flops, (unsigned long long)(flops + oflops + 1));
+        atomicAdd(&A.summary->candidates, (unsigned long long)streamed);
+        if (ninc) atomicAdd(&A.summary->incident_cuts, (unsigned long long)ninc);
+      }
+      if (status & TESS_CELL_DEGENERATE) atomicAdd(&A.summary->degenerate, 1u);
+    }
+    if (A.stats) {
+      tess_cell_stats st;
+      st.streamed = streamed;
+      st.classified = classified;
+      st.cuts = ncuts;
+      st.entries = (uint32_t)e;
+      st.flops = flops + oflops + 1;
+      st.r2 = r2;
+      st.max_vertices = (uint32_t)maxv;
+      st.status = status;
+      st.incident_cuts = ninc;
+      st.reserved = 0;
+      A.stats[oi] = st;
+    }
+    if ((status & TESS_CELL_CAPACITY) && P.redo) P.redo[atomicAdd(P.nredo, 1u)] = item;
+    if ((status & TESS_CELL_TABLE) && P.tredo) P.tredo[atomicAdd(P.ntredo, 1u)] = item;
+  }
+}
+
+// Every block owns one workspace and takes items blockIdx.x, + gridDim.x, ...
+__global__ __launch_bounds__(64) void k_cells_spill(SpillArgs P) {
+  const int lane = threadIdx.x;
+  SpillWs W;
+  spill_ws(P, blockIdx.x, W);
+  typedef const __attribute__((address_space(4))) CellArgs ConstArgs;
+  for (int i = blockIdx.x; i < P.count; i += gridDim.x) {
+    const uint64_t it = P.list64 ? P.list64[i] : (uint64_t)P.list32[i];
+    ConstArgs* cs = (ConstArgs*)(P.sets) + (uint32_t)(it >> 32);
+    spill_cell(*(const CellArgs*)cs, (int)(uint32_t)it, P, W, it, lane);
+    ws_sync();
   }
 }
 
@@ -1435,6 +1956,13 @@ hipError_t launch_cells_geometry(int vpl, const CellArgs& a, bool with_cells, hi
     hipLaunchKernelGGL((k_cells<4, 6, 2, 2>), dim3(nb), dim3(64), 0, s, a);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_cells_spill(const SpillArgs& p, int blocks, hipStream_t s) {
+  if (p.count <= 0) return hipSuccess;
+  if (blocks <= 0 || p.rmax < 24 || p.fmax < 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cells_spill, dim3(blocks), dim3(64), 0, s, p);
   return hipGetLastError();
 }
 

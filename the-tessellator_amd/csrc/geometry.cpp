@@ -32,10 +32,12 @@ struct GeoEvents {
 };
 
 // Per-cell counts (d->geo_count, staged with the cells) to CSR offsets, then
-// the staged vertices and rings of both passes compacted into the CSR arrays.
+// the staged vertices and rings of every pass compacted into the CSR arrays.
 // ev[1] is recorded here; ev[0], when recorded, opens the staging time.
-tess_status finish_geometry(tess_diagram* d, const GeoStaging& first, const GeoStaging& rerun, uint32_t nredo,
-                            GeoEvents& ev, double stage_ms) {
+tess_status finish_geometry(tess_diagram* d, const GeoStage& g, GeoEvents& ev, double stage_ms) {
+  const GeoStaging& first = g.first;
+  const GeoStaging& rerun = g.rerun;
+  const uint32_t nredo = g.nredo;
   tess_ctx* c = d->ctx;
   hipStream_t s = c->stream;
   const size_t n = d->n;
@@ -62,6 +64,8 @@ tess_status finish_geometry(tess_diagram* d, const GeoStaging& first, const GeoS
                   "compact");
   };
   float ms[2] = {(float)stage_ms, 0.f};
+  for (size_t k = 0; k < g.extra.size(); ++k)
+    if ((st = compact(*g.extra[k], g.extra_items[k])) != TESS_OK) return st;
   if ((st = compact(first, n)) != TESS_OK || (nredo && (st = compact(rerun, nredo)) != TESS_OK) ||
       (st = c->hip(hipMemcpyAsync(d->geo_roff.p + F, &d->geo_nr, 8, hipMemcpyHostToDevice, s), "upload")) !=
           TESS_OK ||
@@ -106,7 +110,7 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
     st = compute_cells(d, nullptr);
     d->geo_stage = nullptr;
     if (st != TESS_OK) return st;
-    return finish_geometry(d, g.first, g.rerun, g.nredo, ev, d->tim.cells_ms);
+    return finish_geometry(d, g, ev, d->tim.cells_ms);
   }
   if (!d->computed && (st = compute_cells(d, nullptr)) != TESS_OK) return st;
   if (d->degenerate) return c->fail(TESS_E_DEGENERATE, "the diagram has failed cells");
@@ -125,6 +129,9 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   a.summary = nullptr;
   a.redo = d->redo.p + 1;
   a.nredo = d->redo.p;
+  a.tredo = d->tredo.p + 1;
+  a.ntredo = d->tredo.p;
+  a.defer_table = 1;
   a.geo_count = d->geo_count.p;
   a.geo_slot = g.first.slot.p;
   a.geo_v = g.first.v.p;
@@ -133,6 +140,8 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
   uint32_t nredo = 0;
   if ((st = c->hip(launch_fill_u32(d->geo_count.p, 2 * n, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(d->tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(d->spill.p, 1, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
       (st = c->hip(launch_cells_geometry(1, a, false, s), "geometry")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
@@ -145,8 +154,8 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
     r.work = d->redo_list.p;
     r.nwork = (int)nredo;
     r.xcd_chunk = 0;
-    r.redo = nullptr;
-    r.nredo = nullptr;
+    r.redo = d->spill.p + 1;
+    r.nredo = d->spill.p;
     r.geo_slot = g.rerun.slot.p;
     r.geo_v = g.rerun.v.p;
     r.geo_ring = g.rerun.ring.p;
@@ -156,7 +165,16 @@ tess_status tess_diagram_compute_geometry(tess_diagram* d) {
         (st = c->hip(launch_cells_geometry(4, r, false, s), "geometry")) != TESS_OK)
       return st;
   }
-  return finish_geometry(d, g.first, g.rerun, nredo, ev, -1.0);
+  g.nredo = nredo;
+  // cells past the shell table (staged by longer-table passes) or above 256
+  // vertices (refused)
+  std::vector<uint32_t> tab, spl;
+  if ((st = read_list(c, d->tredo.p, tab)) != TESS_OK || (st = read_list(c, d->spill.p, spl)) != TESS_OK) return st;
+  if (!tab.empty() || !spl.empty()) {
+    g.with_cells = false;
+    if ((st = finish_cells(d, a, std::move(tab), std::move(spl), nullptr, &g)) != TESS_OK) return st;
+  }
+  return finish_geometry(d, g, ev, -1.0);
 }
 
 tess_status tess_diagram_geometry_sizes(const tess_diagram* d, uint64_t* nvertices, uint64_t* nring) {

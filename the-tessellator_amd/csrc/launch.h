@@ -36,8 +36,10 @@ hipError_t launch_gather(const uint32_t* sorted, int n, const double* x, const d
                          const double* z, double* sx, double* sy, double* sz, hipStream_t s);
 hipError_t launch_face_offsets(const uint32_t* fcount, int n, uint64_t* foff, uint64_t* sums,
                                hipStream_t s);
+// cell o's faces are read at snbr[fpos[o]] (fpos null: at o * kFaceStage)
 hipError_t launch_compact(const uint32_t* fcount, const uint64_t* foff, int n, const int32_t* snbr,
-                          const double* sarea, int32_t* nbr, double* area, hipStream_t s);
+                          const double* sarea, int32_t* nbr, double* area, hipStream_t s,
+                          const uint64_t* fpos = nullptr);
 hipError_t launch_invert(const uint32_t* sorted, int n, uint32_t* inv, hipStream_t s);
 
 // cell.hip: vpl = vertex slots per lane (1, 2 or 4)
@@ -58,6 +60,9 @@ hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream
 // (grid = resident blocks: cells_queue_blocks_per_cu() x CUs)
 hipError_t launch_cells_queue(const BatchArgs& b, int grid, hipStream_t s);
 int cells_queue_blocks_per_cu();
+// cells too large for the LDS kernels, one block per workspace (p.wsd ...
+// hold `blocks` workspaces), blocks taking items in turn
+hipError_t launch_cells_spill(const SpillArgs& p, int blocks, hipStream_t s);
 hipError_t launch_diag_arith(const double* a, const double* b, int n, double* sq, double* rc,
                              double* dv, hipStream_t s);
 

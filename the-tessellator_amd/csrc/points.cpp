@@ -19,9 +19,10 @@ struct tess_site_cells {
   double ms[2] = {0, 0};  // cell passes, face offsets + compaction
   bool have_stats = false;
   DevBuf<double> sites, volume, sarea, area;
-  DevBuf<uint32_t> fcount, status, redo, redo_list;
-  DevBuf<int32_t> snbr, nbr;
-  DevBuf<uint64_t> foff, fsums;
+  DevBuf<uint32_t> fcount, status, redo, redo_list, tredo, spill;
+  DevBuf<int32_t> snbr, nbr;  // face arena, CSR
+  DevBuf<uint64_t> foff, fsums, fpos;
+  DevBuf<unsigned long long> arena;
   DevBuf<tess_cell_stats> stats;
   DevBuf<Summary> summary;
 };
@@ -79,12 +80,17 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
       p->status.alloc(m) != hipSuccess || p->snbr.alloc(m * kFaceStage) != hipSuccess ||
       p->sarea.alloc(m * kFaceStage) != hipSuccess || p->foff.alloc(m + 1) != hipSuccess ||
       p->fsums.alloc((m + 1023) / 1024 + 1) != hipSuccess || p->summary.alloc(1) != hipSuccess ||
-      p->redo.alloc(m + 1) != hipSuccess || (p->have_stats && p->stats.alloc(m) != hipSuccess))
+      p->redo.alloc(m + 1) != hipSuccess || p->tredo.alloc(m + 1) != hipSuccess || p->spill.alloc(m + 1) != hipSuccess ||
+      p->fpos.alloc(m) != hipSuccess || p->arena.alloc(1) != hipSuccess ||
+      (p->have_stats && p->stats.alloc(m) != hipSuccess))
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   Events ev;
   if ((st = c->hip(hipMemcpyAsync(p->sites.p, xyz, 24 * m, hipMemcpyHostToDevice, s), "upload")) != TESS_OK ||
       (st = c->hip(hipMemsetAsync(p->summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK ||
-      (st = c->hip(hipMemsetAsync(p->redo.p, 0, sizeof(uint32_t), s), "memset")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(p->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(p->tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(p->spill.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32((uint32_t*)p->arena.p, 2, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK)
     return st;
   CellArgs a = base_args(d, t);
@@ -119,18 +125,24 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
   a.fcount = p->fcount.p;
   a.snbr = p->snbr.p;
   a.sarea = p->sarea.p;
+  a.fpos = p->fpos.p;
+  a.arena_next = p->arena.p;
+  a.arena_cap = (uint64_t)m * kFaceStage;
   a.status = p->status.p;
   a.stats = p->have_stats ? p->stats.p : nullptr;
   a.summary = p->summary.p;
-  a.redo = p->redo.p + 1;
-  a.nredo = p->redo.p;
   const int vpl = pick_vpl(d);
+  // capacity overflows below 256 vertices: the rerun list; at 256: the
+  // spill kernel's; shell-table exhaustions: their own list
+  a.redo = (vpl < 4 ? p->redo.p : p->spill.p) + 1;
+  a.nredo = vpl < 4 ? p->redo.p : p->spill.p;
+  a.tredo = p->tredo.p + 1;
+  a.ntredo = p->tredo.p;
+  a.defer_table = 1;
+  const CellArgs a0 = a;
   if ((st = c->hip(launch_cells(vpl, a, s), "cell kernel")) != TESS_OK) return st;
   uint32_t nredo = 0;
-  Summary sm1;
   if ((st = c->hip(hipMemcpyAsync(&nredo, p->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&sm1, p->summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
-          TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
     return st;
   if (nredo && vpl < 4) {  // sites that outgrew the first pass: the largest variant
@@ -138,17 +150,24 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
     a.work = p->redo_list.p;
     a.nwork = (int)nredo;
     a.xcd_chunk = 0;
-    a.redo = nullptr;
-    a.nredo = nullptr;
+    a.redo = p->spill.p + 1;
+    a.nredo = p->spill.p;
     if ((st = c->hip(hipMemcpyAsync(p->redo_list.p, p->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
                      "redo")) != TESS_OK ||
-        (st = c->hip(hipMemsetAsync(&p->summary.p->capacity, 0, sizeof(unsigned int), s), "memset")) != TESS_OK ||
-        (st = c->hip(launch_cells(4, a, s), "cell kernel")) != TESS_OK ||
-        (st = c->hip(launch_mark_status(p->redo_list.p, (int)nredo, nullptr, p->status.p, a.stats, TESS_CELL_RERUN,
-                                        s),
-                     "mark")) != TESS_OK)
+        (st = c->hip(launch_cells(4, a, s), "cell kernel")) != TESS_OK)
       return st;
   }
+  // sites past the shell table or capacity 256
+  std::vector<uint32_t> tab, spl;
+  if ((st = read_list(c, p->tredo.p, tab)) != TESS_OK || (st = read_list(c, p->spill.p, spl)) != TESS_OK) return st;
+  FinishCounts fin;
+  if ((!tab.empty() || !spl.empty()) &&
+      (st = finish_cells(d, a0, std::move(tab), std::move(spl), &fin)) != TESS_OK)
+    return st;
+  if (nredo && vpl < 4 &&
+      (st = c->hip(launch_mark_status(p->redo_list.p, (int)nredo, nullptr, p->status.p, a.stats, TESS_CELL_RERUN, s),
+                   "mark")) != TESS_OK)
+    return st;
   if ((st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
       (st = c->hip(launch_face_offsets(p->fcount.p, (int)m, p->foff.p, p->fsums.p, s), "face offsets")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&p->total, p->foff.p + m, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
@@ -158,7 +177,8 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   Summary sm2;
   float ms[2] = {0, 0};
-  if ((st = c->hip(launch_compact(p->fcount.p, p->foff.p, (int)m, p->snbr.p, p->sarea.p, p->nbr.p, p->area.p, s),
+  if ((st = c->hip(launch_compact(p->fcount.p, p->foff.p, (int)m, p->snbr.p, p->sarea.p, p->nbr.p, p->area.p, s,
+                                  p->fpos.p),
                    "compact")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[2], s), "event")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&sm2, p->summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
@@ -170,11 +190,7 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
   p->ms[0] = ms[0];
   p->ms[1] = ms[1];
   p->reruns = vpl < 4 ? nredo : 0;
-  // as tess_diagram_compute: overflows of the first pass were counted again
-  // (or not) by the rerun; overflows of the largest variant are failures
-  const uint64_t cap = (nredo && vpl < 4) ? (uint64_t)(sm1.capacity - nredo) + sm2.capacity : sm1.capacity;
-  if (cap) return c->fail(TESS_E_CAPACITY, std::to_string(cap) + " cells exceeded the vertex capacity");
-  if (sm2.table) return c->fail(TESS_E_CAPACITY, std::to_string(sm2.table) + " cells exhausted the shell table");
+  if (fin.failed) return c->fail(TESS_E_CAPACITY, std::to_string(fin.failed) + " cells exceeded the spill workspace limit");
   if (sm2.degenerate)
     return c->fail(TESS_E_DEGENERATE, std::to_string(sm2.degenerate) + " cells failed topology checks");
   *out = p.release();

@@ -77,6 +77,14 @@ struct tess_ctx {
   // shell tables keyed by (cpd, sx, sy, sz, L)
   std::map<std::tuple<int64_t, uint64_t, uint64_t, uint64_t, int>, std::unique_ptr<Table>> tables;
   DevBuf<double> diag;
+  // spill kernel: workspaces (grown on demand), the pass's arguments, lists
+  DevBuf<double> spill_d;
+  DevBuf<uint32_t> spill_u;
+  DevBuf<uint8_t> spill_b;
+  DevBuf<CellArgs> spill_args;
+  DevBuf<uint32_t> fin_list, fin_ctr;
+  DevBuf<uint32_t> fin_tab, fin_cap;
+  DevBuf<uint64_t> fin_tab64, fin_cap64;
 
   tess_status fail(tess_status s, const std::string& m) {
     err = m;
@@ -140,6 +148,11 @@ struct GeoStage {
   uint32_t* count = nullptr;  // [2n]: live vertices, ring entries per cell
   GeoStaging first, rerun;
   uint32_t nredo = 0;
+  // passes with longer shell tables (cells whose security ball reached past
+  // the default table), each staging its own items
+  std::vector<std::unique_ptr<GeoStaging>> extra;
+  std::vector<uint32_t> extra_items;
+  bool with_cells = true;  // the staging passes also write the cell outputs
 };
 
 // Allocates g for `items` work items at capacity 64 vpl and clears its slots.
@@ -174,6 +187,16 @@ struct tess_diagram {
   DevBuf<tess_cell_stats> stats;
   DevBuf<Summary> summary;
   DevBuf<uint32_t> work1, redo_list;
+  // cells that exhausted the shell table / outgrew the largest LDS variant:
+  // count at [0], grid positions from [1]
+  DevBuf<uint32_t> tredo, spill;
+  // face arena: cell o's faces start at arena position fpos[o]; a batch's
+  // sets share the batch's arena (arena_next / arena_cap point there)
+  DevBuf<uint64_t> fpos;
+  DevBuf<unsigned long long> arena_ctr;
+  unsigned long long* arena_next = nullptr;
+  uint64_t arena_cap = 0;
+  uint64_t table_reruns = 0, spill_cells = 0;
   // learned schedule: features, predictions, work lists
   DevBuf<uint16_t> feat;
   DevBuf<float> pred;
@@ -242,5 +265,25 @@ using Planner = std::function<tess_status(tess_diagram*, Schedule&)>;
 tess_status compute_cells(tess_diagram* d, const Planner* plan);
 // XCD run length of whole-set cell passes (the caller's, else 256)
 int plain_xcd_chunk(const tess_diagram* d);
+
+// What finish_cells did.
+struct FinishCounts {
+  uint64_t table = 0;   // cells recomputed with a longer shell table
+  uint64_t spill = 0;   // cells computed by the spill kernel
+  uint64_t failed = 0;  // cells that still failed (workspace limit)
+  int table_l = kTableL;
+};
+// The cells an LDS pass could not finish.  table: work items whose security
+// ball reached past the shell table -- recomputed at capacity 256 with a
+// table twice as long until none is left (the full table has no end);
+// spill: work items above 256 vertices or 128 faces -- the global-memory
+// kernel, its workspace grown until they fit.  `a` is the pass's template
+// (inputs, outputs, filters, site list); items are what its work list
+// holds.  geo: the geometry staging of the pass (table reruns then stage
+// their geometry too; spill cells are refused there).
+tess_status finish_cells(tess_diagram* d, const CellArgs& a, std::vector<uint32_t> table,
+                         std::vector<uint32_t> spill, FinishCounts* out, GeoStage* geo = nullptr);
+// Reads a device list (count at [0], items from [1]) into v.
+tess_status read_list(tess_ctx* c, const uint32_t* dev, std::vector<uint32_t>& v);
 
 }  // namespace tess

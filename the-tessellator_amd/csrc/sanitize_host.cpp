@@ -1,0 +1,80 @@
+// sanitize_host.cpp -- the host runtime's own logic under AddressSanitizer and
+// UndefinedBehaviorSanitizer, on the CPU (no GPU needed): the shell-table
+// builder (celery.rs:418-679: every offset once, keys ascending, ties by
+// generation order, truncated tables a prefix of the full one), the exact
+// integer cube root of the grid size (celery.rs:161-162), and the C ABI's
+// argument checks.  Built with the library's host objects instrumented
+// (`make -C the-tessellator_amd sanitize`); exit status 0 = clean.
+#include <cmath>
+#include <cstdio>
+#include <set>
+#include <vector>
+
+#include "runtime.h"
+
+namespace tess {
+uint64_t generation_index(int64_t cpd, int ox, int oy, int oz);
+}
+
+namespace {
+
+int fails = 0;
+
+void expect(bool ok, const char* what) {
+  if (!ok) {
+    std::printf("FAIL: %s\n", what);
+    ++fails;
+  }
+}
+
+void check_table(int64_t cpd, double sx, double sy, double sz) {
+  std::vector<double> fk, k;
+  std::vector<int32_t> fo, o;
+  double kcut_full = 0, kcut = 0;
+  tess::build_table_host(cpd, sx, sy, sz, (int)(cpd - 1), fk, fo, &kcut_full);
+  const size_t side = (size_t)(2 * cpd - 1);
+  expect(fk.size() == side * side * side, "full table holds every offset");
+  expect(std::isinf(kcut_full), "full table has no cut");
+  std::set<int32_t> seen(fo.begin(), fo.end());
+  expect(seen.size() == fo.size(), "every offset once");
+  for (size_t t = 1; t < fk.size(); ++t) {
+    expect(fk[t - 1] <= fk[t], "keys ascend");
+    if (fk[t - 1] == fk[t]) {
+      int a[3], b[3];
+      tess::unpack_off(fo[t - 1], a[0], a[1], a[2]);
+      tess::unpack_off(fo[t], b[0], b[1], b[2]);
+      expect(tess::generation_index(cpd, a[0], a[1], a[2]) < tess::generation_index(cpd, b[0], b[1], b[2]),
+             "ties in generation order");
+    }
+  }
+  for (int L = 1; L < cpd - 1; L *= 2) {
+    tess::build_table_host(cpd, sx, sy, sz, L, k, o, &kcut);
+    expect(k.size() <= fk.size(), "prefix length");
+    bool prefix = true;
+    for (size_t t = 0; t < k.size(); ++t) prefix = prefix && k[t] == fk[t] && o[t] == fo[t];
+    expect(prefix, "a truncated table is a prefix of the full one");
+    expect(k.size() == fk.size() || fk[k.size()] >= kcut, "the first entry past the prefix is at or above kcut");
+  }
+}
+
+}  // namespace
+
+int main() {
+  check_table(5, 0.2, 0.2, 0.2);
+  check_table(9, 0.11, 0.37, 0.05);
+  check_table(21, 1.0 / 21, 1.0 / 21, 1.0 / 21);
+  const double cubes[] = {1, 7, 8, 9, 26, 27, 28, 999, 1000, 1001, 9261, 9261.000001, 804357, 1e15};
+  const int64_t roots[] = {1, 1, 2, 2, 2, 3, 3, 9, 10, 10, 21, 21, 93, 100000};
+  for (size_t i = 0; i < sizeof(cubes) / sizeof(cubes[0]); ++i)
+    expect(tess::icbrt_floor(cubes[i]) == roots[i], "integer cube root");
+  expect(tess::icbrt_floor(0.5) == 0 && tess::icbrt_floor(std::nan("")) == 0, "cube root below 1 and of NaN");
+  // the C ABI refuses bad arguments without touching a device
+  tess_opts o;
+  tess_opts_default(&o);
+  expect(o.cell_density == 1.25, "default density");
+  expect(tess_ctx_create(0, nullptr) == TESS_E_INVALID, "null out");
+  expect(tess_diagram_compute(nullptr) == TESS_E_INVALID, "null diagram");
+  expect(std::string(tess_last_error(nullptr)) == "null context", "error text of a null context");
+  std::printf("%s\n", fails ? "sanitized host run: FAILURES" : "sanitized host run: clean");
+  return fails ? 1 : 0;
+}

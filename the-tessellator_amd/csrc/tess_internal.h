@@ -34,9 +34,12 @@ struct Summary {
   unsigned int capacity;          // cells with TESS_CELL_CAPACITY
   unsigned int table;             // cells with TESS_CELL_TABLE
   unsigned int fallback;          // learned mode: cells that fell back
+  unsigned long long incident_cuts;  // cuts with no Inside vertex next to the Outside region
 };
 
-// Faces staged per cell before CSR compaction.
+// Faces staged per cell before CSR compaction when a pass writes per-cell
+// slots (single-cell queries, geometry staging); whole-set passes write
+// their faces into a shared arena instead (CellArgs::fpos).
 constexpr int kFaceStage = 64;
 
 // Learned search-radius model 27 -> 64 -> 64 -> 2.  Parameter vector layout
@@ -78,13 +81,21 @@ struct CellArgs {
   // outputs, indexed by original point index
   double* volume;
   uint32_t* fcount;
-  int32_t* snbr;           // [n][kFaceStage]
-  double* sarea;           // [n][kFaceStage]
+  int32_t* snbr;           // face arena (fpos set) or per-cell slots [n][kFaceStage]
+  double* sarea;
+  uint64_t* fpos;          // nullable: cell o's faces start at snbr[fpos[o]], a
+                           // range reserved by one atomic on arena_next (no
+                           // per-cell face limit); null: at o * kFaceStage
+  unsigned long long* arena_next;
+  uint64_t arena_cap;      // faces the arena holds
   uint32_t* status;
   tess_cell_stats* stats;  // nullable
   Summary* summary;        // nullable
   uint32_t* redo;          // grid positions of cells needing a larger variant
   uint32_t* nredo;
+  uint32_t* tredo;         // nullable: cells whose security ball reaches past
+  uint32_t* ntredo;        // the shell table (rerun with a longer one)
+  int defer_table;         // 1: such cells write no outputs (a rerun will)
   // geometry of whole sets (the geometry kernel only), staged per work item
   // w: vertices geo_v[w][64 VPL][3], rings geo_ring[w][half-edge capacity],
   // ring offsets in the cell geo_roff[w][kFaceStage]; geo_count[o] = live
@@ -111,8 +122,33 @@ struct BatchArgs {
   int chunk;              // items taken per atomic
   uint64_t* redo;         // capacity overflows (nullable)
   uint32_t* nredo;
+  uint64_t* tredo;        // shell-table exhaustions (nullable)
+  uint32_t* ntredo;
   int xcd_chunk;          // > 0: XCDs take runs of this many consecutive items
 };
+
+// Cells too large for the LDS kernels (more than 256 vertices or 128
+// faces): one wavefront per cell with the polyhedron as face rings in a
+// global-memory workspace of rmax ring entries and fmax faces per block.
+struct SpillArgs {
+  const CellArgs* sets;    // device: per-set arguments (a diagram: one)
+  const uint32_t* list32;  // items of set 0 (grid positions / site indices)
+  const uint64_t* list64;  // or (set << 32) | position
+  int count;
+  int rmax, fmax;
+  double* wsd;             // per block: two ring buffers [rmax][3], segments [fmax][6]
+  uint32_t* wsu;           // per block: face offsets 2 x [fmax + 1], tags 2 x [fmax],
+                           // segment links [fmax], face terms [fmax][4]
+  uint8_t* wsb;            // per block: ring-entry classes [rmax], segment flags [fmax]
+  uint64_t* redo;          // items that outgrew the workspace
+  uint32_t* nredo;
+  uint64_t* tredo;         // items that exhausted the shell table
+  uint32_t* ntredo;
+};
+// Workspace of one spill block.
+__host__ __device__ inline size_t spill_wsd_doubles(int rmax, int fmax) { return 6ull * rmax + 6ull * fmax; }
+__host__ __device__ inline size_t spill_wsu_words(int fmax) { return 2ull * (fmax + 1) + 7ull * fmax; }
+__host__ __device__ inline size_t spill_wsb_bytes(int rmax, int fmax) { return (size_t)rmax + (size_t)fmax; }
 
 // Radius queries over one grid (query.hip).
 struct QueryArgs {

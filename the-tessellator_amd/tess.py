@@ -23,6 +23,7 @@ STATUS = {0: "TESS_OK", 1: "TESS_E_INVALID", 2: "TESS_E_NOMEM", 3: "TESS_E_DEVIC
           4: "TESS_E_CAPACITY", 5: "TESS_E_DEGENERATE", 6: "TESS_E_STATE", 7: "TESS_E_NOTFOUND"}
 MODE_PLAIN, MODE_LEARNED = 0, 1
 CELL_CAPACITY, CELL_TABLE, CELL_DEGENERATE, CELL_FALLBACK, CELL_RERUN = 1, 2, 4, 8, 16
+CELL_LONG_TABLE, CELL_SPILL = 32, 64
 
 EXPORTS = [
     "tess_opts_default", "tess_ctx_create", "tess_ctx_destroy", "tess_last_error", "tess_ctx_stream",
@@ -65,23 +66,26 @@ class Opts(C.Structure):
 class CellStats(C.Structure):
     _fields_ = [("streamed", C.c_uint32), ("classified", C.c_uint32), ("cuts", C.c_uint32),
                 ("entries", C.c_uint32), ("flops", C.c_uint64), ("r2", C.c_double),
-                ("max_vertices", C.c_uint32), ("status", C.c_uint32)]
+                ("max_vertices", C.c_uint32), ("status", C.c_uint32), ("incident_cuts", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 STATS_DTYPE = np.dtype([("streamed", "<u4"), ("classified", "<u4"), ("cuts", "<u4"), ("entries", "<u4"),
-                        ("flops", "<u8"), ("r2", "<f8"), ("max_vertices", "<u4"), ("status", "<u4")])
+                        ("flops", "<u8"), ("r2", "<f8"), ("max_vertices", "<u4"), ("status", "<u4"),
+                        ("incident_cuts", "<u4"), ("reserved", "<u4")])
 assert STATS_DTYPE.itemsize == C.sizeof(CellStats)
 
 
 class Timings(C.Structure):
     _fields_ = [("bin_ms", C.c_double), ("cells_ms", C.c_double), ("compact_ms", C.c_double),
                 ("total_ms", C.c_double), ("table_ms", C.c_double), ("flops", C.c_uint64),
-                ("fallback_cells", C.c_uint64), ("capacity_reruns", C.c_uint64), ("plan_ms", C.c_double)]
+                ("fallback_cells", C.c_uint64), ("capacity_reruns", C.c_uint64), ("plan_ms", C.c_double),
+                ("table_reruns", C.c_uint64), ("spill_cells", C.c_uint64), ("incident_cuts", C.c_uint64)]
 
 
 class BatchConfig(C.Structure):
     _fields_ = [("first_capacity", C.c_int32), ("xcd_chunk", C.c_int32), ("redo_chunk", C.c_int32),
-                ("learned", C.c_int32)]
+                ("learned", C.c_int32), ("eager", C.c_int32)]
 
 
 _lib = None
