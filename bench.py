@@ -38,6 +38,17 @@ Workloads (--workload):
                the 1,000,000 points of the clustered set (seed 25, grid of
                about 5 points per cell), positions and radii resident in
                HBM, results left in HBM; reports queries/s (its own metric).
+  fcc16m       16 perturbed FCC lattices of 63^3 cells (1,000,188 points
+               each; cubic lattice constant a = 1, nearest-neighbour distance
+               a / sqrt 2; Gaussian jitter sigma 1e-3; seeds 4000-4015) in
+               one batch with the learned order (model trained on two other
+               lattices, seeds 4100-4101); the plain batch is timed beside
+               it and the learned path's fallback rate is reported.
+
+The roofline of the cell passes is FP64 VALU issue: the geometry is single
+rounded add/sub/mul/div/sqrt with FMA contraction off, so the no-FMA
+ceiling (39.3 Tops/s, half the 78.6 TFLOP/s FMA-counted peak) is reported
+beside the published peak.
 """
 import argparse
 import json
@@ -59,13 +70,16 @@ import tess  # noqa: E402
 # counted as two operations.  The cell kernel issues no FMA (contraction is
 # off for bit-exact parity), so every counted op is one instruction lane-op.
 FP64_PEAK_TFLOPS = 78.6
+FP64_NOFMA_TOPS = 39.3
 HBM_PEAK_GBS = 8000.0
 METRIC = "Voronoi cells/sec (f64, 1 MI355X): plain vs learned-radius batched kernel"
 
 WORKLOADS = {
-    "clustered4m": dict(desc="learned-radius batched kernel, 4,000,000 clustered points (64 Gaussians sigma 0.02 "
-                             "+ 20% uniform) in [0,1]^3, f64, box walls",
-                        points=4_000_000, seed=25, density=5.0, kind="clustered", train_seeds=(20, 21, 22, 23, 24)),
+    "clustered4m": dict(desc="4,000,000 clustered points (64 Gaussians sigma 0.02 + 20% uniform) in [0,1]^3, f64, "
+                             "box walls: batched path with the autotuned order (learned or list), plain kernel "
+                             "timed beside",
+                        points=4_000_000, seed=25, density=5.0, kind="clustered", train_seeds=(20, 21, 22, 23, 24),
+                        cpu_frac=0.01),
     "uniform1m": dict(desc="plain kernel, 1,000,000 points i.i.d. uniform in [0,1]^3, f64, box walls",
                       points=1_000_000, seed=1, density=5.0, kind="uniform"),
     "batch512": dict(desc="batched kernel, 512 uniform sets, N log-uniform [1e3, 2e5], box sides U[0.5, 2]",
@@ -80,6 +94,10 @@ WORKLOADS = {
     "queries1m": dict(desc="real-radius neighbour queries (r = 0.01) at the 1,000,000 points of the clustered set "
                            "(64 Gaussians sigma 0.02 + 20% uniform, seed 25), f64",
                       points=1_000_000, seed=25, density=5.0, kind="queries", radius=0.01),
+    "fcc16m": dict(desc="learned + batched kernel, 16 perturbed FCC lattices of 63^3 cells (1,000,188 points each, "
+                        "cubic constant a = 1, jitter sigma 1e-3) in [0,63]^3, f64, box walls",
+                   lattices=16, ncell=63, seed=4000, train_seeds=(4100, 4101), density=5.0, kind="fcc",
+                   cpu_frac=0.002),
 }
 
 
@@ -93,20 +111,64 @@ def make_points(w):
     return tess.clustered(w["points"], w["seed"])
 
 
-def cpu_baseline(pts, lo, hi, density, sample, threads):
-    """Time the CPU restatement (oracle) on a random sample of the same cells."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def roofline_fp64(achieved, kernel, flops, kernel_ms):
+    """FP64 VALU roofline fields of a cell pass (achieved in Tops/s)."""
+    return {
+        "bound": "valu",
+        "ceiling": "FP64 VALU: published peak 78.6 TFLOP/s counts an FMA as 2; the geometry issues only "
+                   "single-rounded add/sub/mul/div/sqrt (contraction off, no MFMA), so its own ceiling is "
+                   "39.3 Tops/s (frac_no_fma)",
+        "kernel": kernel,
+        "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "frac_no_fma": achieved / FP64_NOFMA_TOPS,
+        "flops_per_launch": flops,
+        "kernel_ms": kernel_ms,
+        "traffic": None,
+    }
+
+
+def cpu_baseline(pts, lo, hi, density, frac, threads):
+    """The CPU restatement (oracle, C++ -O3, OpenMP dynamic over cells) on a
+    seeded random sample of the same cells, grid build included: best of 3
+    on `threads` host threads and best of 3 on one."""
     from oracle import oracle as orc
 
     rng = np.random.default_rng(0)
-    cells = np.sort(rng.choice(len(pts), min(sample, len(pts)), replace=False)).astype(np.uint32)
-    r = orc.compute(pts, lo, hi, density=density, cells=cells, threads=threads)
-    return len(cells) / r["seconds"], r["seconds"], len(cells)
+    k = max(1, int(round(frac * len(pts))))
+    cells = np.sort(rng.choice(len(pts), k, replace=False)).astype(np.uint32)
+    best = {}
+    for t in (threads, 1):
+        secs = []
+        for _ in range(3):
+            r = orc.compute(pts, lo, hi, density=density, cells=cells, threads=t)
+            secs.append(r["seconds"])
+        best[t] = min(secs)
+    return {
+        "value": k / best[threads], "unit": "cells/s", "cores": threads, "kind": "port",
+        "single_core": k / best[1],
+        "sample": f"{k} random cells ({100 * frac:.0f}% of the {len(pts)}-point set, seed 0 choice), grid build "
+                  f"included, best of 3: {best[threads]:.2f} s on {threads} threads, {best[1]:.2f} s on 1",
+        "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+    }
 
 
 def fixed_config(text):
-    """'64,0,4,1' -> first_capacity, xcd_chunk, redo_chunk, learned."""
+    """'64,0,4,1[,1]' -> first_capacity, xcd_chunk, redo_chunk, learned[, eager]."""
     v = [int(x) for x in text.split(",")]
-    return dict(zip(("first_capacity", "xcd_chunk", "redo_chunk", "learned"), v))
+    return dict(zip(("first_capacity", "xcd_chunk", "redo_chunk", "learned", "eager"), v))
 
 
 def timed(fn, steps, warmup, ctx):
@@ -149,6 +211,16 @@ def bench_clustered(ctx, w, args):
     pts = make_points(w)
     n = len(pts)
     lo, hi = np.zeros(3), np.ones(3)
+    if args.plain_only:
+        # the plain kernel alone (profiling runs): no model, no batch
+        d = tess.Diagram(ctx, pts, lo, hi, density=w["density"], xcd_chunk=args.xcd_chunk)
+
+        def plain_only_step():
+            d.compute()
+            return d.timings()
+
+        ms, tims = timed(plain_only_step, args.steps, args.warmup, ctx)
+        return pts, lo, hi, n, ms, tims, d.cells(), {}, {"mode": "plain"}
     path = os.path.join(tempfile.gettempdir(), "tess_clustered4m_model.bin")
     if args.model:
         m = tess.Model(ctx)
@@ -211,7 +283,10 @@ def bench_clustered(ctx, w, args):
         "config_sweep": sweep,
         "model": train,
     }
-    return pts, lo, hi, n, ms, tims, got, extra, {"mode": "learned-batched"}
+    mode = "learned-batched" if cfg["learned"] else "batched (the tuner kept the plain list order)"
+    if not cfg["learned"]:
+        extra["fallback_cells"] = extra["fallback_rate"] = None
+    return pts, lo, hi, n, ms, tims, got, extra, {"mode": mode}
 
 
 def bench_uniform(ctx, w, args):
@@ -263,6 +338,71 @@ def bench_batch(ctx, w, args):
              "tuned_config": cfg}
     lo, hi = sets[0][1], sets[0][2]
     return sets[0][0], lo, hi, n, ms, tims, None, extra, {"mode": "batched", "sets": len(sets)}
+
+
+def end_to_end(ctx, pts, lo, hi, density, reps=3):
+    """The plain path from host memory to host memory: upload (create), the
+    compute, and the download of every cell's results (PCIe both ways)."""
+    best = None
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        d = tess.Diagram(ctx, pts, lo, hi, density=density)
+        d.compute()
+        d.cells()
+        t1 = time.perf_counter()
+        d.close()
+        best = t1 - t0 if best is None else min(best, t1 - t0)
+    return {"ms": best * 1e3, "cells_per_s": len(pts) / best,
+            "includes": "host points -> HBM, compute, every cell's results HBM -> host (best of 3)"}
+
+
+def bench_fcc(ctx, w, args):
+    """16 jittered FCC lattices in one batch, learned order (model trained on
+    two other lattices), the plain batch timed beside it."""
+    box = np.full(3, float(w["ncell"]))
+    lat = [tess.fcc(w["seed"] + i, w["ncell"]) for i in range(w["lattices"])]
+    n = sum(len(p) for p in lat)
+    m = tess.Model(ctx, seed=3)
+    t0 = time.perf_counter()
+    for s in w["train_seeds"]:
+        d = tess.Diagram(ctx, tess.fcc(s, w["ncell"]), np.zeros(3), box, density=w["density"], collect_stats=True)
+        d.compute()
+        m.add_samples(d)
+        d.close()
+    loss = m.train(epochs=2, batch=8192, lr=2e-3, seed=1)
+    train = dict(samples=m.sample_count(), seconds=time.perf_counter() - t0, loss=[float(x) for x in loss])
+    sets = [(p, np.zeros(3), box) for p in lat]
+    plain = tess.Batch(ctx, sets, density=w["density"])
+
+    def plain_step():
+        plain.compute()
+        return plain.timings()
+
+    plain_ms, plain_t = timed(plain_step, max(2, args.steps // 2), 1, ctx)
+    b = tess.Batch(ctx, sets, density=w["density"])
+    b.set_config(learned=1)
+
+    def step():
+        b.compute(m)
+        return b.timings()
+
+    ms, tims = timed(step, args.steps, args.warmup, ctx)
+    identical = all(np.array_equal(b.cells(s)[k], v) for s in range(len(sets)) for k, v in plain.cells(s).items())
+    extra = {
+        "plain": {"value": n / (plain_ms / 1e3), "ms_per_step": plain_ms,
+                  "phases_ms": {k: mean(plain_t, k) for k in ("bin_ms", "cells_ms", "compact_ms")}},
+        "speedup_vs_plain": plain_ms / ms,
+        "bitwise_equal_to_plain": bool(identical),
+        "fallback_cells": int(tims[-1]["fallback_cells"]),
+        "fallback_rate": tims[-1]["fallback_cells"] / n,
+        "capacity_reruns": int(tims[-1]["capacity_reruns"]),
+        "model": train,
+    }
+    plain.close()
+    cells = {"face_count": np.concatenate([b.cells(s)["face_count"] for s in range(len(sets))])}
+    b.close()
+    return lat[0], np.zeros(3), box, n, ms, tims, cells, extra, {"mode": "learned-batched", "lattices": len(sets)}
 
 
 def bench_geometry(ctx, w, args):
@@ -338,21 +478,15 @@ def bench_geometry(ctx, w, args):
         "cells_and_geometry_ms": {"compute_then_geometry": float(np.mean(sep)),
                                   "one_call_staged_by_compute": float(np.mean(one))},
         "sizes": {"vertices": nv.value, "ring_entries": nr.value, "faces": int(d.cells()["face_count"].sum())},
-        "roofline": {
-            "bound": "mfma",
-            "ceiling": "FP64 78.6 TFLOP/s (the staging pass recomputes every cell)",
-            "kernel": "staging pass k_cells<1, 2, 1, 1> (+ its capacity rerun), HIP events on the context "
-                      "stream",
-            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops, "kernel_ms": stage_ms,
-            "traffic": None,
-        },
+        "roofline": roofline_fp64(achieved, "staging pass k_cells<1, 2, 1, 1> (+ its capacity rerun; it "
+                                            "recomputes every cell), HIP events on the context stream", flops,
+                                  stage_ms),
     }
     if not args.no_cpu:
         from oracle import oracle as orc
 
         threads = min(16, os.cpu_count() or 1)
-        k = min(n, args.cpu_sample)
+        k = max(1, int(args.cpu_frac * n))
         cells = np.sort(np.random.default_rng(0).choice(n, k, replace=False)).astype(np.uint32)
         t0 = time.perf_counter()
         orc.compute_geometry(pts, np.zeros(3), np.ones(3), density=w["density"], cells=cells, threads=threads)
@@ -419,14 +553,8 @@ def bench_sites(ctx, w, args):
                        "note": "host sites uploaded over PCIe and result buffers allocated inside the call"},
         "capacity_reruns": tims[-1]["reruns"],
         "faces": faces,
-        "roofline": {
-            "bound": "mfma",
-            "ceiling": "FP64 78.6 TFLOP/s (vector = matrix peak), FMA counted as 2; no MFMA in the clip",
-            "kernel": "cell passes k_cells<1, 2, 1, 0> + capacity rerun, HIP events on the context stream",
-            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_launch": flops, "kernel_ms": cells_ms,
-            "traffic": None,
-        },
+        "roofline": roofline_fp64(achieved, "cell passes k_cells<1, 2, 1, 0> + capacity rerun, HIP events on "
+                                            "the context stream", flops, cells_ms),
     }
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
@@ -443,7 +571,7 @@ def bench_sites(ctx, w, args):
         from oracle import oracle as orc
 
         threads = min(16, os.cpu_count() or 1)
-        k = min(m, int(2.5 * args.cpu_sample))  # a site costs about a fifth of a clustered cell
+        k = max(1, int(args.cpu_frac * m))
         sel = np.random.default_rng(0).choice(m, k, replace=False)
         t0 = time.perf_counter()
         orc.compute_at_points(pts, sites[sel], lo, hi, density=w["density"], threads=threads)
@@ -538,7 +666,7 @@ def bench_queries(ctx, w, args):
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        m = min(args.cpu_sample, 200_000)
+        m = max(1, int(args.cpu_frac * n))
         sel = np.random.default_rng(0).choice(n, m, replace=False)
         t0 = time.perf_counter()
         off, _ = orc.queries(tess.QUERY_REAL_RADIUS, pts, pts[sel], r, density=w["density"])
@@ -558,7 +686,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clustered4m", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--cpu-frac", type=float, default=None,
+                    help="fraction of the cells the CPU baseline times (seeded random sample; default per "
+                         "workload, 1-5%%)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--vcap", type=int, default=0, help="vertex capacity of the plain kernel (0: automatic)")
     ap.add_argument("--xcd-chunk", type=int, default=0,
@@ -567,12 +697,16 @@ def main():
     ap.add_argument("--batch-config", default="", help="skip the autotuner: first_capacity,xcd_chunk,redo_chunk,"
                                                        "learned (e.g. 64,0,4,0)")
     ap.add_argument("--sweep", action="store_true", help="clustered4m: also time a fixed set of batch configurations")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="clustered4m: time the plain kernel only (no model, no batch), e.g. under a profiler")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path; --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
 
     w = WORKLOADS[args.workload]
+    if args.cpu_frac is None:
+        args.cpu_frac = w.get("cpu_frac", 0.05)
     ctx = tess.Context(0)
     if w["kind"] == "queries":
         bench_queries(ctx, w, args)
@@ -583,7 +717,7 @@ def main():
     if w["kind"] == "sites":
         bench_sites(ctx, w, args)
         return
-    fn = {"clustered": bench_clustered, "uniform": bench_uniform, "batch": bench_batch}[w["kind"]]
+    fn = {"clustered": bench_clustered, "uniform": bench_uniform, "batch": bench_batch, "fcc": bench_fcc}[w["kind"]]
     pts, lo, hi, n, ms, tims, cells, extra, cfg_extra = fn(ctx, w, args)
     value = n / (ms / 1e3)
     kernel_ms = mean(tims, "cells_ms")
@@ -606,19 +740,8 @@ def main():
         "config": dict({"workload": w["desc"], "points": n, "cell_density": w["density"]}, **cfg_extra),
         "phases_ms": {"bin": mean(tims, "bin_ms"), "plan": mean(tims, "plan_ms"), "cells": kernel_ms,
                       "compact": mean(tims, "compact_ms")},
-        "roofline": {
-            "bound": "mfma",
-            "ceiling": "FP64 (vector = matrix peak) 78.6 TFLOP/s, FMA counted as 2; the geometry issues "
-                       "only single-rounded add/sub/mul/div/sqrt on the vector units, no MFMA",
-            "kernel": "cell passes (first pass + capacity rerun + face scan), HIP events on the context stream",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "flops_per_launch": flops,
-            "kernel_ms": kernel_ms,
-            "traffic": None,
-        },
+        "roofline": roofline_fp64(achieved, "cell passes (first pass + capacity rerun), HIP events on the "
+                                            "context stream", flops, kernel_ms),
     }
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
@@ -641,14 +764,11 @@ def main():
         # volume + face count + status (16 B), 12 B per staged face
         line["roofline"]["alg_hbm_gbs"] = (44 * n + 12 * faces) / (kernel_ms / 1e3) / 1e9
     line.update(extra)
+    if w["kind"] in ("clustered", "uniform"):
+        line["end_to_end"] = end_to_end(ctx, pts, lo, hi, w["density"])
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        rate, sec, cnt = cpu_baseline(pts, lo, hi, w["density"], args.cpu_sample, threads)
-        line["cpu_baseline"] = {
-            "value": rate, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"{cnt} random cells of the {len(pts)}-point set (seed 0 choice), grid build included, "
-                      f"{sec:.1f} s on {threads} threads of {os.cpu_count()} visible",
-        }
+        line["cpu_baseline"] = cpu_baseline(pts, lo, hi, w["density"], args.cpu_frac, threads)
     print(json.dumps(line), flush=True)
 
 
