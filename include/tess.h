@@ -413,6 +413,27 @@ tess_status tess_batch_copy_cells(const tess_batch* b, size_t set, double* volum
                                   uint64_t* face_offset, int32_t* neighbor, double* face_area);
 tess_status tess_batch_timings(const tess_batch* b, tess_timings* out);
 
+/* ---- without device allocation (no counterpart in the reference) ------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  The cells of n
+ * device points (AoS xyz, n <= TESS_NOALLOC_MAX_POINTS, a grid of at most 32
+ * cells per dimension) in the box [lo, hi], as tess_diagram_create_device +
+ * tess_diagram_compute + tess_diagram_device_cells would give them, bitwise,
+ * written into the caller's device buffers volume[n], face_count[n],
+ * face_offset[n + 1], neighbor[face_cap], face_area[face_cap]; *total_faces
+ * receives the face count (past face_cap: TESS_E_CAPACITY, neighbor and
+ * face_area untouched).  Every temporary lives in one fixed device array of
+ * the library, TESS_NOALLOC_SCRATCH_BYTES per GPU, which each GPU holds from
+ * the library's first use until the process ends.  Calls on one GPU share
+ * it: calls that overlap on one GPU race (one context and its stream at a
+ * time per device).  Larger inputs are refused with TESS_E_INVALID. */
+#define TESS_NOALLOC_MAX_POINTS 16384
+#define TESS_NOALLOC_SCRATCH_BYTES 15459416
+tess_status tess_compute_noalloc(tess_ctx* ctx, const double* xyz_device, size_t n, const double lo[3],
+                                 const double hi[3], double cell_density, double* volume, uint32_t* face_count,
+                                 uint64_t* face_offset, int32_t* neighbor, double* face_area, uint64_t face_cap,
+                                 uint64_t* total_faces);
+
 /* ---- diagnostics (no counterpart in the reference) --------------------- */
 /* Device f64 sqrt, reciprocal and division of n inputs (for checking that
  * they are correctly rounded like the host's). */

@@ -462,6 +462,7 @@ tess_status tess_ctx_create(int dev, tess_ctx** out) {
   tess_ctx* c = new (std::nothrow) tess_ctx;
   if (!c) return TESS_E_NOMEM;
   c->device = dev;
+  (void)noalloc_setup(dev);  // the scratch address of this device, once
   if (hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return TESS_E_DEVICE;

@@ -62,6 +62,10 @@ struct Table {
 void build_table_host(int64_t cpd, double sx, double sy, double sz, int L, std::vector<double>& key,
                       std::vector<int32_t>& off, double* kcut);
 constexpr int kTableL = 12;
+// Position of an offset in the generation loops of celery.rs:437-673, and its
+// key (sq(gx sx) + sq(gy sy)) + sq(gz sz) of the gaps (celery.rs:419-427).
+uint64_t generation_index(int64_t cpd, int ox, int oy, int oz);
+double shell_key(int ox, int oy, int oz, double sx, double sy, double sz);
 uint64_t bits(double v);
 int64_t icbrt_floor(double x);
 
@@ -283,6 +287,9 @@ struct FinishCounts {
 // their geometry too; spill cells are refused there).
 tess_status finish_cells(tess_diagram* d, const CellArgs& a, std::vector<uint32_t> table,
                          std::vector<uint32_t> spill, FinishCounts* out, GeoStage* geo = nullptr);
+// The no-allocation entry point's scratch address on `device` (noalloc.hip),
+// taken once per device with that device current.
+tess_status noalloc_setup(int device);
 // Reads a device list (count at [0], items from [1]) into v.
 tess_status read_list(tess_ctx* c, const uint32_t* dev, std::vector<uint32_t>& v);
 

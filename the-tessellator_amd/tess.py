@@ -24,6 +24,7 @@ STATUS = {0: "TESS_OK", 1: "TESS_E_INVALID", 2: "TESS_E_NOMEM", 3: "TESS_E_DEVIC
 MODE_PLAIN, MODE_LEARNED = 0, 1
 CELL_CAPACITY, CELL_TABLE, CELL_DEGENERATE, CELL_FALLBACK, CELL_RERUN = 1, 2, 4, 8, 16
 CELL_LONG_TABLE, CELL_SPILL = 32, 64
+NOALLOC_MAX_POINTS = 16384
 
 EXPORTS = [
     "tess_opts_default", "tess_ctx_create", "tess_ctx_destroy", "tess_last_error", "tess_ctx_stream",
@@ -44,7 +45,7 @@ EXPORTS = [
     "tess_site_cells_total_faces", "tess_site_cells_copy", "tess_site_cells_copy_stats", "tess_site_cells_timings",
     "tess_site_cells_destroy", "tess_diagram_compute_geometry", "tess_diagram_geometry_sizes",
     "tess_diagram_copy_geometry", "tess_diagram_geometry_timings", "tess_diagram_copy_index_maps",
-    "tess_cell_at_particle",
+    "tess_cell_at_particle", "tess_compute_noalloc",
 ]
 QUERY_CELLS, QUERY_CELL_RADIUS, QUERY_REAL_RADIUS, QUERY_CLOUD = 0, 1, 2, 3
 ORDER_INPUT, ORDER_MORTON = 0, 1
@@ -158,6 +159,8 @@ def load_library(path: str | None = None) -> C.CDLL:
         "tess_batch_copy_cells": (C.c_int, [vp, sz, dp, dp, dp, dp, dp]),
         "tess_batch_timings": (C.c_int, [vp, C.POINTER(Timings)]),
         "tess_diag_arith": (C.c_int, [vp, dp, dp, sz, dp, dp, dp]),
+        "tess_compute_noalloc": (C.c_int, [vp, vp, sz, dp, dp, C.c_double, vp, vp, vp, vp, vp, C.c_uint64,
+                                           C.POINTER(C.c_uint64)]),
         "tess_diagram_query": (C.c_int, [vp, C.c_int, dp, sz, dp, sz, C.c_int64, C.POINTER(vp)]),
         "tess_diagram_query_device": (C.c_int, [vp, C.c_int, vp, sz, vp, sz, C.c_int64, C.POINTER(vp)]),
         "tess_diagram_cells_at_particles": (C.c_int, [vp, dp, sz, C.c_double, C.c_int64, C.POINTER(vp)]),
