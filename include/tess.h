@@ -105,10 +105,12 @@ enum {
                                 ball reached past the default 25^3-offset prefix
                                 (the reference streams all (2 cpd - 1)^3 offsets,
                                 celery.rs:418-679, :971-1018) */
-  TESS_CELL_SPILL = 64u      /* more than 256 vertices or 128 faces: computed by
+  TESS_CELL_SPILL = 64u,     /* more than 256 vertices or 128 faces: computed by
                                 the global-memory kernel (no size limit but the
                                 workspace, grown on demand like the reference's
                                 Pool, pool.rs:85-110) */
+  TESS_CELL_ARENA = 128u     /* no room left in the face arena: the call fails
+                                with TESS_E_CAPACITY */
 };
 
 /* Device pointers to a computed diagram's results (valid until destroy). */
@@ -137,6 +139,8 @@ typedef struct {
   uint64_t spill_cells;    /* cells computed by the global-memory spill kernel */
   uint64_t incident_cuts;  /* cuts bordered by Incident vertices only (see
                               tess_cell_stats.incident_cuts) */
+  uint64_t planned_long;   /* learned mode: cells routed to the long-search pass
+                              by their predicted radius */
 } tess_timings;
 
 void tess_opts_default(tess_opts* opts);
@@ -428,7 +432,7 @@ tess_status tess_batch_timings(const tess_batch* b, tess_timings* out);
  * it: calls that overlap on one GPU race (one context and its stream at a
  * time per device).  Larger inputs are refused with TESS_E_INVALID. */
 #define TESS_NOALLOC_MAX_POINTS 16384
-#define TESS_NOALLOC_SCRATCH_BYTES 15459416
+#define TESS_NOALLOC_SCRATCH_BYTES 15460688
 tess_status tess_compute_noalloc(tess_ctx* ctx, const double* xyz_device, size_t n, const double lo[3],
                                  const double hi[3], double cell_density, double* volume, uint32_t* face_count,
                                  uint64_t* face_offset, int32_t* neighbor, double* face_area, uint64_t face_cap,

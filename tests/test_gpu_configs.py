@@ -112,7 +112,16 @@ def test_clustered_4m_learned_equals_plain(ctx, cfg2):
     for k in cells:
         assert np.array_equal(got[k], cells[k]), k
     t = d.timings()
-    log(f"learned: {t['total_ms']:.1f} ms, fallback rate {t['fallback_cells'] / len(pts):.5f}")
+    # what the trained network gains over the method without it, on this set
+    plain = tess.Diagram(ctx, pts, LO, HI, density=DENSITY)
+    plain.compute()
+    pm = min(plain.compute() or plain.timings()["total_ms"] for _ in range(3))
+    lm = min(d.compute_learned(m) or d.timings()["total_ms"] for _ in range(3))
+    rate = t["fallback_cells"] / len(pts)
+    log(f"plain {len(pts) / pm * 1e3 / 1e6:.2f}M cells/s, learned {len(pts) / lm * 1e3 / 1e6:.2f}M cells/s "
+        f"(gain {pm / lm:.3f}x), fallback rate {rate:.5f}, routed long {t['planned_long']}")
+    assert 0.0 <= rate < 0.05
+    plain.close()
     b = tess.Batch(ctx, [(pts, LO, HI)], density=DENSITY)
     b.set_config(learned=1)
     b.compute(m)

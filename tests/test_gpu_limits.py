@@ -181,3 +181,37 @@ def test_batch_with_long_tables_and_spills(ctx, eager):
     assert t["table_reruns"] > 0 and t["spill_cells"] >= 1
     for s, (p, lo, hi) in enumerate(sets):
         same_cells(b.cells(s), orc.compute(p, lo, hi))
+
+
+def test_learned_radius_routes_long_searches(ctx):
+    """The model's predicted final radius R sends the cells whose security
+    ball will reach past the default table (4 R^2 >= kcut) straight to a pass
+    with a table long enough for the largest such prediction; the others run
+    as planned, and a cell the prediction under-sized still gets its exact
+    longer-table rerun (counted as a fallback).  Trained on three sets of the
+    same kind (20,000 uniform points in a [-1, 2]^3 container) by the GPU
+    kernels; outputs and counters bitwise equal to the plain path."""
+    lo, hi = np.full(3, -1.0), np.full(3, 2.0)
+    m = tess.Model(ctx, seed=5)
+    for s in (2101, 2102, 2103):
+        t = tess.Diagram(ctx, tess.uniform(20000, s), lo, hi, collect_stats=True)
+        t.compute()
+        m.add_samples(t)
+        t.close()
+    m.train(epochs=150, batch=2048, lr=2e-3, seed=1)
+    pts = tess.uniform(20000, 2104)
+    plain = tess.Diagram(ctx, pts, lo, hi, collect_stats=True)
+    plain.compute()
+    d = tess.Diagram(ctx, pts, lo, hi, collect_stats=True)
+    d.compute_learned(m)
+    same_cells(d.cells(), plain.cells())
+    same_counters(d.stats(), plain.stats())
+    tp, tl = plain.timings(), d.timings()
+    print(f"plain: {tp['table_reruns']} long-table reruns, {tp['total_ms']:.2f} ms; learned: "
+          f"{tl['planned_long']} cells routed long, {tl['table_reruns']} reruns, {tl['fallback_cells']} fallbacks, "
+          f"{tl['total_ms']:.2f} ms")
+    assert tl["planned_long"] > 0
+    assert tl["table_reruns"] < tp["table_reruns"]
+    st = d.stats()["status"]
+    flagged = int(((st & tess.CELL_FALLBACK) != 0).sum())
+    assert flagged <= tl["fallback_cells"] and (flagged > 0) == (tl["fallback_cells"] > 0)

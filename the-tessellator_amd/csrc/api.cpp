@@ -129,14 +129,14 @@ tess_status alloc_all(tess_diagram* d) {
             (d->opts.point_order != TESS_ORDER_MORTON || d->codes.alloc(n) == hipSuccess) &&
             d->sz.alloc(n) == hipSuccess && d->volume.alloc(n) == hipSuccess &&
             d->fcount.alloc(n) == hipSuccess && d->status.alloc(n) == hipSuccess &&
-            d->redo.alloc(n + 1) == hipSuccess && d->snbr.alloc(n * kFaceStage) == hipSuccess &&
-            d->sarea.alloc(n * kFaceStage) == hipSuccess && d->foff.alloc(n + 1) == hipSuccess &&
+            d->redo.alloc(n + 1) == hipSuccess && d->snbr.alloc(arena_capacity(n)) == hipSuccess &&
+            d->sarea.alloc(arena_capacity(n)) == hipSuccess && d->foff.alloc(n + 1) == hipSuccess &&
             d->fsums.alloc((n + 1023) / 1024 + 1) == hipSuccess && d->work1.alloc(1) == hipSuccess &&
             d->summary.alloc(1) == hipSuccess && d->tredo.alloc(n + 1) == hipSuccess &&
             d->spill.alloc(n + 1) == hipSuccess && d->fpos.alloc(n) == hipSuccess &&
-            d->arena_ctr.alloc(1) == hipSuccess;
+            d->arena_ctr.alloc(kArenaWords) == hipSuccess;
   d->arena_next = d->arena_ctr.p;
-  d->arena_cap = (uint64_t)n * kFaceStage;
+  d->arena_cap = arena_capacity(n);
   if (ok && d->opts.collect_stats) ok = d->stats.alloc(n) == hipSuccess;
   if (!ok) return c->fail(TESS_E_NOMEM, "device allocation failed");
   return TESS_OK;
@@ -361,8 +361,15 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
       (st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(launch_fill_u32(d->tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(launch_fill_u32(d->spill.p, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32((uint32_t*)d->arena_next, 2, 0u, s), "fill")) != TESS_OK)
+      (st = c->hip(launch_fill_u32((uint32_t*)d->arena_next, 2 * kArenaWords, 0u, s), "fill")) != TESS_OK)
     return st;
+  // the planner's long-search list first (capacity 256, a table sized from
+  // the largest predicted radius), then the capacity lists, largest first
+  if (sch.count[3]) {
+    Table* tl = c->table(d->cpd, G.sx, G.sy, G.sz, sch.long_l, &st);
+    if (!tl) return st;
+    if ((st = run_cells(d, tl, 4, sch.list[3], sch.count[3], true)) != TESS_OK) return st;
+  }
   static const int kVpl[3] = {1, 2, 4};
   for (int slot = 2; slot >= 0; --slot)
     if (sch.count[slot] && (st = run_cells(d, t, kVpl[slot], sch.list[slot], sch.count[slot], true)) != TESS_OK)
@@ -387,10 +394,18 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   std::vector<uint32_t> tab, spl;
   if ((st = read_list(c, d->tredo.p, tab)) != TESS_OK || (st = read_list(c, d->spill.p, spl)) != TESS_OK) return st;
   FinishCounts fin;
+  const int ntab = (int)tab.size();
   if (!tab.empty() || !spl.empty()) {
     CellArgs a = base_args(d, t);
     if ((st = finish_cells(d, a, std::move(tab), std::move(spl), &fin, d->geo_stage)) != TESS_OK) return st;
   }
+  // planned passes: a cell whose search outran its table was under-sized by
+  // the prediction too (marked after its rerun wrote its status)
+  if (plan && ntab &&
+      (st = c->hip(launch_mark_status(d->tredo.p + 1, ntab, d->sorted.p, d->status.p,
+                                      d->opts.collect_stats ? d->stats.p : nullptr, TESS_CELL_FALLBACK, s),
+                   "mark")) != TESS_OK)
+    return st;
   if (nredo && (st = c->hip(launch_mark_status(d->redo_list.p, (int)nredo, d->sorted.p, d->status.p,
                                                d->opts.collect_stats ? d->stats.p : nullptr,
                                                TESS_CELL_RERUN | (plan ? TESS_CELL_FALLBACK : 0u), s),
@@ -402,10 +417,11 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   uint64_t total = 0;
   unsigned long long used = 0;
   if ((st = c->hip(hipMemcpyAsync(&total, d->foff.p + n, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&used, d->arena_next, 8, hipMemcpyDeviceToHost, s), "arena")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&used, d->arena_next + (size_t)kArenaStride * (kArenaParts + 1), 8, hipMemcpyDeviceToHost, s),
+                   "arena")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK)
     return st;
-  if (used > d->arena_cap) return c->fail(TESS_E_CAPACITY, "the face arena is full");
+  if (used) return c->fail(TESS_E_CAPACITY, "the face arena is full");
   if (d->nbr.alloc(total) != hipSuccess || d->area.alloc(total) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   if ((st = c->hip(launch_compact(d->fcount.p, d->foff.p, n, d->snbr.p, d->sarea.p, d->nbr.p, d->area.p, s,
@@ -425,7 +441,8 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   d->tim.total_ms = ev.ms(0, 4);
   d->tim.table_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   d->tim.capacity_reruns = nredo;
-  d->tim.fallback_cells = plan ? nredo : 0;
+  d->tim.fallback_cells = plan ? nredo + (uint64_t)ntab : 0;
+  d->tim.planned_long = sch.count[3];
   d->tim.flops = sm2.flops;
   d->tim.incident_cuts = sm2.incident_cuts;
   d->table_reruns = d->tim.table_reruns = fin.table;

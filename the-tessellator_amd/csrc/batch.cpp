@@ -107,10 +107,10 @@ struct tess_batch {
 
 namespace {
 
-// ctrl words: [0] rerun count, [1] rerun counter, [2] binning check, [3..6]
-// slot bounds of the learned order, [8] shell-table exhaustions, [9] cells
-// for the spill kernel
-constexpr int kCtrlRedo = 0, kCtrlQueue = 1, kCtrlBad = 2, kCtrlBounds = 3, kCtrlTab = 8, kCtrlSpill = 9,
+// ctrl words: [0] rerun count, [1] rerun counter, [2] binning check, [3..8]
+// slot bounds of the learned order (and its radius word), [10] shell-table
+// exhaustions, [11] cells for the spill kernel
+constexpr int kCtrlRedo = 0, kCtrlQueue = 1, kCtrlBad = 2, kCtrlBounds = 3, kCtrlTab = 10, kCtrlSpill = 11,
               kCtrlWords = 12;
 
 tess_status make_views(tess_batch* b) {
@@ -144,11 +144,11 @@ tess_status make_views(tess_batch* b) {
     d->fcount.view(b->fcount.p + p, d->n);
     d->status.view(b->status.p + p, d->n);
     // every set writes into the batch's face arena
-    d->snbr.view(b->snbr.p, b->N * kFaceStage);
-    d->sarea.view(b->sarea.p, b->N * kFaceStage);
+    d->snbr.view(b->snbr.p, arena_capacity(b->N));
+    d->sarea.view(b->sarea.p, arena_capacity(b->N));
     d->fpos.view(b->fpos.p + p, d->n);
     d->arena_next = b->arena.p;
-    d->arena_cap = (uint64_t)b->N * kFaceStage;
+    d->arena_cap = arena_capacity(b->N);
     d->summary.view(b->summary.p + s, 1);
     if (b->opts.collect_stats) d->stats.view(b->stats.p + p, d->n);
     for (int k = 0; k < 3; ++k) {
@@ -316,7 +316,7 @@ tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32
   hipStream_t s = c->stream;
   tess_status st;
   if (launch_fill_u32(b->ctrl.p, 2, 0u, s) != hipSuccess || launch_fill_u32(b->ctrl.p + kCtrlTab, 2, 0u, s) != hipSuccess ||
-      launch_fill_u32((uint32_t*)b->arena.p, 2, 0u, s) != hipSuccess ||
+      launch_fill_u32((uint32_t*)b->arena.p, 2 * kArenaWords, 0u, s) != hipSuccess ||
       launch_fill_u32((uint32_t*)b->summary.p, b->nsets * (sizeof(Summary) / 4), 0u, s) != hipSuccess)
     return c->fail(TESS_E_DEVICE, "fill");
   BatchArgs a;
@@ -478,13 +478,14 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
   (void)hipEventRecord(ev[3], s);
   if ((st = c->hip(hipMemcpyAsync(&mail->total, b->foff.p + b->N, 8, hipMemcpyDeviceToHost, s), "total")) !=
           TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&mail->arena, b->arena.p, 8, hipMemcpyDeviceToHost, s), "arena")) != TESS_OK ||
+      (st = c->hip(hipMemcpyAsync(&mail->arena, b->arena.p + (size_t)kArenaStride * (kArenaParts + 1), 8, hipMemcpyDeviceToHost, s),
+                   "arena")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(sm.data(), b->summary.p, sizeof(Summary) * b->nsets, hipMemcpyDeviceToHost, s),
                    "summary")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "cells")) != TESS_OK)
     return st;
   const uint64_t total = mail->total;
-  if (mail->arena > (uint64_t)b->N * kFaceStage) return c->fail(TESS_E_CAPACITY, "the face arena is full");
+  if (mail->arena) return c->fail(TESS_E_CAPACITY, "the face arena is full");
   if (b->nbr.alloc(total) != hipSuccess || b->area.alloc(total) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   if ((st = c->hip(launch_compact(b->fcount.p, b->foff.p, (int)b->N, b->snbr.p, b->sarea.p, b->nbr.p, b->area.p, s,
@@ -598,13 +599,13 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
             b->cursor.alloc(C) == hipSuccess && b->big.alloc(C) == hipSuccess && b->delim.alloc(B) == hipSuccess &&
             b->sums.alloc(S) == hipSuccess && b->misc.alloc(4 * nsets) == hipSuccess &&
             b->volume.alloc(N) == hipSuccess && b->fcount.alloc(N) == hipSuccess && b->status.alloc(N) == hipSuccess &&
-            b->snbr.alloc(N * kFaceStage) == hipSuccess && b->sarea.alloc(N * kFaceStage) == hipSuccess &&
+            b->snbr.alloc(arena_capacity(N)) == hipSuccess && b->sarea.alloc(arena_capacity(N)) == hipSuccess &&
             b->foff.alloc(N + 1) == hipSuccess && b->fsums.alloc((N + 1023) / 1024 + 1) == hipSuccess &&
             b->items.alloc(N) == hipSuccess && b->redo.alloc(N) == hipSuccess &&
             b->base_dev.alloc(nsets + 1) == hipSuccess && b->dend_dev.alloc(nsets) == hipSuccess &&
             b->summary.alloc(nsets) == hipSuccess && b->args.alloc(nsets) == hipSuccess &&
             b->ctrl.alloc(kCtrlWords) == hipSuccess && b->fpos.alloc(N) == hipSuccess &&
-            b->tredo.alloc(N) == hipSuccess && b->spill.alloc(N) == hipSuccess && b->arena.alloc(1) == hipSuccess &&
+            b->tredo.alloc(N) == hipSuccess && b->spill.alloc(N) == hipSuccess && b->arena.alloc(kArenaWords) == hipSuccess &&
             hipHostMalloc((void**)&b->mail, sizeof(Mail), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc((void**)&b->hgrid_pin, sizeof(GridDev) * nsets, hipHostMallocDefault) == hipSuccess;
   if (ok && b->opts.collect_stats) ok = b->stats.alloc(N) == hipSuccess;

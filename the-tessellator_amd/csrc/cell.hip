@@ -202,6 +202,30 @@ __device__ __forceinline__ double box_gap(int b, double vmin, double size, doubl
 
 __device__ __forceinline__ double stop_thr(double r2) { return (4.0 * r2) * (1.0 + 1e-9); }
 
+// nf consecutive face slots of the arena: in the region of this wave's XCD,
+// else in the shared overflow region; ~0 (and the no-room count bumped)
+// when both are full.  Wave-uniform result.
+__device__ __forceinline__ uint64_t arena_reserve(unsigned long long* ctr, uint64_t cap, uint32_t nf, int lane) {
+  unsigned long long p = ~0ull;
+  if (lane == 0) {
+    const uint64_t part = arena_part_size(cap);
+    const uint32_t x = blockIdx.x % kArenaParts;
+    unsigned long long q = atomicAdd(ctr + (size_t)kArenaStride * x, (unsigned long long)nf);
+    if (q + nf <= part) {
+      p = (unsigned long long)x * part + q;
+    } else {
+      q = atomicAdd(ctr + (size_t)kArenaStride * kArenaParts, (unsigned long long)nf);
+      if (kArenaParts * part + q + nf <= cap)
+        p = kArenaParts * part + q;
+      else
+        atomicAdd(ctr + (size_t)kArenaStride * (kArenaParts + 1), 1ull);
+    }
+  }
+  const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p & 0xffffffffull));
+  const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p >> 32));
+  return ((uint64_t)phi << 32) | plo;
+}
+
 // A chunk with at least this many staged candidates is pre-tested lane-parallel.
 constexpr int kPreMin = 16;
 
@@ -1078,18 +1102,14 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   // cell's own slots
   uint64_t fbase = (uint64_t)oi * kFaceStage;
   if (GEO != 1 && A.fpos != nullptr && !table_rerun && (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0) {
-    unsigned long long p = 0;
-    if (lane == 0) p = atomicAdd(A.arena_next, (unsigned long long)nfaces);
-    const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p & 0xffffffffull));
-    const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p >> 32));
-    fbase = ((uint64_t)phi << 32) | plo;
-    if (fbase + (uint64_t)nfaces > A.arena_cap)
-      status |= TESS_CELL_CAPACITY;  // arena full (the caller reports it)
+    fbase = arena_reserve(A.arena_next, A.arena_cap, (uint32_t)nfaces, lane);
+    if (fbase == ~0ull)
+      status |= TESS_CELL_ARENA;  // arena full (counted; the caller reports it)
     else if (lane == 0)
       A.fpos[oi] = fbase;
   }
   uint64_t oflops = 0;
-  if ((status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) == 0 && !table_rerun) {
+  if ((status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_ARENA)) == 0 && !table_rerun) {
     double area[FPL], contrib[FPL];
     int32_t tagv[FPL];
     int seqv[FPL];
@@ -1278,7 +1298,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   oflops = wave_sum_u64(oflops);
   if (lane == 0) {
     if (GEO != 1) {
-      if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) {
+      if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_ARENA)) {
         A.volume[oi] = 0.0;
         A.fcount[oi] = 0;
       }
@@ -1756,17 +1776,13 @@ __device__ void spill_cell(const CellArgs& A, const int gpos, const SpillArgs& P
   bool wrote = false;
   if (!fail) {
     if (A.fpos != nullptr) {
-      unsigned long long p = 0;
-      if (lane == 0) p = atomicAdd(A.arena_next, (unsigned long long)nf);
-      const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p & 0xffffffffull));
-      const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(p >> 32));
-      fbase = ((uint64_t)phi << 32) | plo;
-      if (fbase + (uint64_t)nf > A.arena_cap) status |= TESS_CELL_CAPACITY;
+      fbase = arena_reserve(A.arena_next, A.arena_cap, (uint32_t)nf, lane);
+      if (fbase == ~0ull) status |= TESS_CELL_ARENA;
     } else if (nf > kFaceStage) {
       status |= TESS_CELL_CAPACITY;
     }
   }
-  if (!fail && !(status & TESS_CELL_CAPACITY)) {
+  if (!fail && !(status & (TESS_CELL_CAPACITY | TESS_CELL_ARENA))) {
     // per face: the fan from the lexicographically smallest vertex
     // (polyhedron.rs:776-808), its volume term and area (interface.rs:408-410)
     const double* R = W.ring[b];

@@ -67,17 +67,27 @@ hipError_t launch_diag_arith(const double* a, const double* b, int n, double* sq
                              double* dv, hipStream_t s);
 
 // sched.hip: work lists for the learned schedule
-constexpr int kSchedBuckets = 64;             // cost buckets per capacity slot
-constexpr int kSchedKeys = 3 * kSchedBuckets;  // slot-major keys
+constexpr int kSchedBuckets = 64;                      // cost buckets per slot
+constexpr int kSchedSlots = 4;                         // capacity 64, 128, 256; long search
+constexpr int kSchedKeys = kSchedSlots * kSchedBuckets;  // slot-major keys (fit a byte)
+static_assert(kSchedKeys <= 256, "schedule keys are bytes");
+// The long-search slot: a cell goes there when y[o * stride] * scale + shift
+// (its predicted log(R / s_min)) exceeds threshold (y null: no such slot).
+struct SchedRadius {
+  const float* y = nullptr;
+  float scale = 1.f, shift = 0.f, threshold = 0.f;
+};
 // Key of every grid position from a per-point cost v = y[o * stride] * scale
 // + shift (log(1 + candidates)), o = sorted[k] (k when sorted is null):
-// capacity slot by thresholds t1 < t2, then
-// descending cost.  Writes the stable key order to list[n] and the start of
-// each slot (and n) to bounds[4].  hist: kSchedKeys * ceil(n / 256) entries;
-// sums: ceil(hist / 1024) + 1.
+// capacity slot by thresholds t1 < t2 (or the long-search slot, radius),
+// then descending cost.  Writes the stable key order to list[n], the start
+// of each slot (and n) to bounds[0..kSchedSlots] and the largest predicted
+// log radius of the long-search slot (order-preserving u32 bits, 0 when the
+// slot is empty) to bounds[kSchedSlots + 1].  hist: kSchedKeys * ceil(n /
+// 256) entries; sums: ceil(hist / 1024) + 1.
 hipError_t launch_schedule(const float* y, int stride, float scale, float shift, float t1, float t2,
                            const uint32_t* sorted, int n, uint8_t* key, uint32_t* hist, uint32_t* sums,
-                           uint32_t* list, uint32_t* bounds, hipStream_t s);
+                           uint32_t* list, uint32_t* bounds, hipStream_t s, const SchedRadius* radius = nullptr);
 // batches: item list, per-item costs from per-point predictions, binning check
 hipError_t launch_batch_items(const uint64_t* base, int nsets, uint64_t total, uint64_t* items, hipStream_t s);
 hipError_t launch_batch_costs(const uint64_t* items, uint64_t total, const uint64_t* base, const uint32_t* sorted,

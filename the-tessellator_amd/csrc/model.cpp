@@ -72,31 +72,60 @@ tess_status infer(tess_diagram* d, const tess_model* m) {
 }
 
 // Work lists from per-point costs y[o * stride] * scale + shift.
+// Shell rings a table needs so that its cut key exceeds the stop threshold of
+// a cell of radius r: kcut(L) = min_a (L s_a)^2 > 4 r^2 (1 + 1e-9), with a
+// 5% margin on r; the default table when that is enough, the full order at
+// most.
+int rings_for_radius(const tess_diagram* d, double r) {
+  const double smin = smallest_edge(d);
+  int L = kTableL;
+  const int64_t full = d->cpd - 1;
+  while (L < full) {
+    const double kc = (L * smin) * (L * smin);
+    if (kc > 4.0 * (1.05 * r) * (1.05 * r) * (1.0 + 1e-9)) break;
+    L = (int)std::min<int64_t>(2 * (int64_t)L, full);
+  }
+  return L;
+}
+
 tess_status schedule(tess_diagram* d, const float* y, int stride, float scale, float shift, float t1, float t2,
-                     Schedule& sch) {
+                     Schedule& sch, const SchedRadius* radius = nullptr) {
   tess_ctx* c = d->ctx;
   const int n = (int)d->n;
   const size_t ntiles = (d->n + 255) / 256;
   const size_t m = (size_t)kSchedKeys * ntiles;
   if (d->sched_key.alloc((d->n + 3) / 4) != hipSuccess || d->sched_hist.alloc(m) != hipSuccess ||
-      d->sched_sums.alloc((m + 1023) / 1024 + 1) != hipSuccess || d->sched_list.alloc(d->n + 4) != hipSuccess)
+      d->sched_sums.alloc((m + 1023) / 1024 + 1) != hipSuccess ||
+      d->sched_list.alloc(d->n + kSchedSlots + 2) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   uint32_t* bounds_dev = d->sched_list.p + d->n;
   tess_status st;
   if ((st = c->hip(launch_schedule(y, stride, scale, shift, t1, t2, d->sorted.p, n, (uint8_t*)d->sched_key.p,
-                                   d->sched_hist.p, d->sched_sums.p, d->sched_list.p, bounds_dev, c->stream),
+                                   d->sched_hist.p, d->sched_sums.p, d->sched_list.p, bounds_dev, c->stream,
+                                   radius),
                    "schedule")) != TESS_OK)
     return st;
-  uint32_t b[4];
+  uint32_t b[kSchedSlots + 2];
   if ((st = c->hip(hipMemcpyAsync(b, bounds_dev, sizeof(b), hipMemcpyDeviceToHost, c->stream), "schedule")) !=
           TESS_OK ||
       (st = c->hip(hipStreamSynchronize(c->stream), "schedule")) != TESS_OK)
     return st;
-  if (!(b[0] == 0 && b[0] <= b[1] && b[1] <= b[2] && b[2] <= b[3] && b[3] == (uint32_t)n))
-    return c->fail(TESS_E_DEVICE, "schedule lost cells");
-  for (int slot = 0; slot < 3; ++slot) {
+  bool ok = b[0] == 0 && b[kSchedSlots] == (uint32_t)n;
+  for (int slot = 0; slot < kSchedSlots; ++slot) ok = ok && b[slot] <= b[slot + 1];
+  if (!ok) return c->fail(TESS_E_DEVICE, "schedule lost cells");
+  for (int slot = 0; slot < kSchedSlots; ++slot) {
     sch.list[slot] = d->sched_list.p + b[slot];
     sch.count[slot] = (int)(b[slot + 1] - b[slot]);
+  }
+  if (sch.count[3]) {
+    // the table of the long-search slot: long enough for its largest
+    // predicted radius (order-preserving bits back to the float)
+    const uint32_t u = b[kSchedSlots + 1];
+    uint32_t bitsv = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float w;
+    std::memcpy(&w, &bitsv, 4);
+    const double r = std::exp((double)w) * smallest_edge(d);
+    sch.long_l = rings_for_radius(d, std::isfinite(r) ? r : 1e300);
   }
   return TESS_OK;
 }
@@ -383,7 +412,17 @@ tess_status tess_diagram_compute_learned(tess_diagram* d, const tess_model* m) {
   Planner plan = [m](tess_diagram* dd, Schedule& sch) -> tess_status {
     tess_status st = infer(dd, m);
     if (st != TESS_OK) return st;
-    return schedule(dd, dd->pred.p + 1, kMlpOut, m->norm[3], m->norm[2], m->calib[0], m->calib[1], sch);
+    // the predicted final radius R routes a cell to the long-search list when
+    // its security ball would reach past the default table: 4 R^2 (1 + 1e-9)
+    // >= kcut = (12 s_min)^2, i.e. log(R / s_min) >= log 6, less a margin of
+    // 20% for the prediction's error; a cell it under-sizes still finds its
+    // table by the exact rerun
+    SchedRadius r;
+    r.y = dd->pred.p;
+    r.scale = m->norm[1];
+    r.shift = m->norm[0];
+    r.threshold = (float)(std::log(kTableL / 2.0) + std::log(0.8));
+    return schedule(dd, dd->pred.p + 1, kMlpOut, m->norm[3], m->norm[2], m->calib[0], m->calib[1], sch, &r);
   };
   return compute_cells(d, &plan);
 }

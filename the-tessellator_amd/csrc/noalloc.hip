@@ -50,7 +50,7 @@ struct NaArena {
   double spill_d[kNaSpillBlocks * (6 * kNaRmax + 6 * kNaFmax)];
   uint64_t fpos[kNaPoints];
   uint64_t fsums[kNaPoints / 1024 + 2];
-  unsigned long long arena_next;
+  unsigned long long arena_ctr[kArenaWords];
   uint64_t spill_fail[kNaPoints];
   GridDev grid;
   Summary summary;
@@ -213,11 +213,12 @@ extern "C" tess_status tess_compute_noalloc(tess_ctx* ctx, const double* dxyz, s
   uint32_t* spill = NA(A, spill);
   uint32_t* ctr = NA(A, ctr);
   Summary* summary = field<Summary>(A, offsetof(NaArena, summary));
-  unsigned long long* arena_next = field<unsigned long long>(A, offsetof(NaArena, arena_next));
+  unsigned long long* arena_next = NA(A, arena_ctr);
   if ((st = ctx->hip(launch_fill_u32(redo, 1, 0u, s), "fill")) != TESS_OK ||
       (st = ctx->hip(launch_fill_u32(spill, 1, 0u, s), "fill")) != TESS_OK ||
       (st = ctx->hip(launch_fill_u32(ctr, 4, 0u, s), "fill")) != TESS_OK ||
-      (st = ctx->hip(launch_fill_u32(reinterpret_cast<uint32_t*>(arena_next), 2, 0u, s), "fill")) != TESS_OK ||
+      (st = ctx->hip(launch_fill_u32(reinterpret_cast<uint32_t*>(arena_next), 2 * kArenaWords, 0u, s), "fill")) !=
+          TESS_OK ||
       (st = ctx->hip(launch_fill_u32(reinterpret_cast<uint32_t*>(summary), sizeof(Summary) / 4, 0u, s), "fill")) !=
           TESS_OK)
     return st;
@@ -248,7 +249,7 @@ extern "C" tess_status tess_compute_noalloc(tess_ctx* ctx, const double* dxyz, s
   a.sarea = NA(A, sarea);
   a.fpos = NA(A, fpos);
   a.arena_next = arena_next;
-  a.arena_cap = (uint64_t)n * kNaFaces;
+  a.arena_cap = (uint64_t)kNaPoints * kNaFaces;  // the whole scratch arena
   a.status = NA(A, status);
   a.summary = summary;
   a.redo = redo + 1;
@@ -304,13 +305,14 @@ extern "C" tess_status tess_compute_noalloc(tess_ctx* ctx, const double* dxyz, s
   Summary sm;
   if ((st = ctx->hip(launch_face_offsets(face_count, ni, face_offset, NA(A, fsums), s), "face offsets")) != TESS_OK ||
       (st = ctx->hip(hipMemcpyAsync(&total, face_offset + n, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
-      (st = ctx->hip(hipMemcpyAsync(&used, arena_next, 8, hipMemcpyDeviceToHost, s), "arena")) != TESS_OK ||
+      (st = ctx->hip(hipMemcpyAsync(&used, arena_next + (size_t)kArenaStride * (kArenaParts + 1), 8, hipMemcpyDeviceToHost, s),
+                     "arena")) != TESS_OK ||
       (st = ctx->hip(hipMemcpyAsync(&sm, summary, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
           TESS_OK ||
       (st = ctx->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK)
     return st;
-  if (used > a.arena_cap)
-    return ctx->fail(TESS_E_CAPACITY, "more than " + std::to_string(kNaFaces) + " faces per point on average");
+  if (used)
+    return ctx->fail(TESS_E_CAPACITY, "the face arena (" + std::to_string(kNaFaces) + " faces per point) is full");
   *total_faces = total;
   if (sm.degenerate)
     return ctx->fail(TESS_E_DEGENERATE, std::to_string(sm.degenerate) + " cells failed topology checks");

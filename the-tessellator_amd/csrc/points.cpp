@@ -77,11 +77,11 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
   p->m = m;
   p->have_stats = d->opts.collect_stats != 0;
   if (p->sites.alloc(3 * m) != hipSuccess || p->volume.alloc(m) != hipSuccess || p->fcount.alloc(m) != hipSuccess ||
-      p->status.alloc(m) != hipSuccess || p->snbr.alloc(m * kFaceStage) != hipSuccess ||
-      p->sarea.alloc(m * kFaceStage) != hipSuccess || p->foff.alloc(m + 1) != hipSuccess ||
+      p->status.alloc(m) != hipSuccess || p->snbr.alloc(arena_capacity(m)) != hipSuccess ||
+      p->sarea.alloc(arena_capacity(m)) != hipSuccess || p->foff.alloc(m + 1) != hipSuccess ||
       p->fsums.alloc((m + 1023) / 1024 + 1) != hipSuccess || p->summary.alloc(1) != hipSuccess ||
       p->redo.alloc(m + 1) != hipSuccess || p->tredo.alloc(m + 1) != hipSuccess || p->spill.alloc(m + 1) != hipSuccess ||
-      p->fpos.alloc(m) != hipSuccess || p->arena.alloc(1) != hipSuccess ||
+      p->fpos.alloc(m) != hipSuccess || p->arena.alloc(kArenaWords) != hipSuccess ||
       (p->have_stats && p->stats.alloc(m) != hipSuccess))
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   Events ev;
@@ -90,7 +90,7 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
       (st = c->hip(launch_fill_u32(p->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(launch_fill_u32(p->tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(launch_fill_u32(p->spill.p, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32((uint32_t*)p->arena.p, 2, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32((uint32_t*)p->arena.p, 2 * kArenaWords, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK)
     return st;
   CellArgs a = base_args(d, t);
@@ -127,7 +127,7 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
   a.sarea = p->sarea.p;
   a.fpos = p->fpos.p;
   a.arena_next = p->arena.p;
-  a.arena_cap = (uint64_t)m * kFaceStage;
+  a.arena_cap = arena_capacity(m);
   a.status = p->status.p;
   a.stats = p->have_stats ? p->stats.p : nullptr;
   a.summary = p->summary.p;

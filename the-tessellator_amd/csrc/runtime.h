@@ -257,10 +257,12 @@ tess_status create_common(tess_ctx* ctx, size_t n, const double lo[3], const dou
 // push_redo is set.
 tess_status run_cells(tess_diagram* d, Table* t, int vpl, const uint32_t* work, int nwork, bool push_redo);
 // Work lists of grid positions per vertex capacity (slot 0: 64, 1: 128,
-// 2: 256); a null list with a count means every cell.
+// 2: 256) and the long-search list (slot 3: capacity 256 with a shell table
+// of long_l rings); a null list with a count means every cell.
 struct Schedule {
-  const uint32_t* list[3] = {nullptr, nullptr, nullptr};
-  int count[3] = {0, 0, 0};
+  const uint32_t* list[4] = {nullptr, nullptr, nullptr, nullptr};
+  int count[4] = {0, 0, 0, 0};
+  int long_l = 0;
 };
 // Fills a schedule after binning (learned mode).
 using Planner = std::function<tess_status(tess_diagram*, Schedule&)>;

@@ -1,8 +1,9 @@
 // sched.hip -- work lists of the learned schedule.
 //
-// Every grid position gets a key: its vertex-capacity slot (from the
-// predicted candidate count and two thresholds), then a cost bucket in
-// descending predicted cost.  A stable counting sort by key (tiles of 256
+// Every grid position gets a key: its slot -- a vertex capacity (from the
+// predicted candidate count and two thresholds), or the long-search slot
+// when the predicted final radius reaches past the default shell table --
+// then a cost bucket in descending predicted cost.  A stable counting sort by key (tiles of 256
 // positions, per-wavefront rank by ballot matching) turns the keys into one
 // list per slot, heaviest cells first and grid order inside a bucket, so the
 // dispatcher starts the long cells early and neighbouring cells still run
@@ -16,10 +17,24 @@ namespace {
 
 constexpr int kTile = 256;
 
+// Order-preserving u32 image of a float (larger float, larger integer).
+__device__ __forceinline__ uint32_t ordered(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 __device__ __forceinline__ uint32_t sched_key(const float* y, int stride, float scale, float shift, float t1, float t2,
-                                              uint32_t o) {
+                                              const SchedRadius& r, uint32_t o, float* vr) {
   const float v = y[(size_t)o * stride] * scale + shift;  // o: point of the cell
-  const int slot = v > t2 ? 2 : (v > t1 ? 1 : 0);
+  int slot = v > t2 ? 2 : (v > t1 ? 1 : 0);
+  *vr = -INFINITY;
+  if (r.y) {  // predicted log(R / s_min) past the threshold: the long-search slot
+    const float w = r.y[(size_t)o * stride] * r.scale + r.shift;
+    if (w > r.threshold) {
+      slot = 3;
+      *vr = w;
+    }
+  }
   // 4 buckets per unit of log(1 + C), heaviest first; NaN lands in bucket 0
   float u = (16.0f - v) * 4.0f;
   u = fminf(fmaxf(u, 0.0f), (float)(kSchedBuckets - 1));
@@ -38,10 +53,10 @@ __device__ __forceinline__ uint64_t match_key(uint32_t key, bool valid) {
 }
 
 __global__ __launch_bounds__(kTile) void k_sched_hist(const float* __restrict__ y, int stride, float scale,
-                                                      float shift, float t1, float t2,
+                                                      float shift, float t1, float t2, SchedRadius r,
                                                       const uint32_t* __restrict__ sorted, int n,
                                                       uint8_t* __restrict__ keys, uint32_t* __restrict__ hist,
-                                                      int ntiles) {
+                                                      int ntiles, uint32_t* __restrict__ rmax) {
   __shared__ uint32_t cnt[kTile / 64][kSchedKeys];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < (kTile / 64) * kSchedKeys; i += kTile) (&cnt[0][0])[i] = 0;
@@ -49,10 +64,15 @@ __global__ __launch_bounds__(kTile) void k_sched_hist(const float* __restrict__ 
   const int k = blockIdx.x * kTile + t;
   const bool valid = k < n;
   uint32_t key = 0;
+  float vr = -INFINITY;
   if (valid) {
-    key = sched_key(y, stride, scale, shift, t1, t2, sorted ? sorted[k] : (uint32_t)k);
+    key = sched_key(y, stride, scale, shift, t1, t2, r, sorted ? sorted[k] : (uint32_t)k, &vr);
     keys[k] = (uint8_t)key;
   }
+  // the largest predicted radius of the long-search slot
+  float wmax = vr;
+  for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
+  if (lane == 0 && wmax > -INFINITY) atomicMax(rmax, ordered(wmax));
   const uint64_t m = match_key(key, valid);
   if (valid && lane == __ffsll((unsigned long long)m) - 1) cnt[w][key] = (uint32_t)__popcll(m);
   __syncthreads();
@@ -84,11 +104,11 @@ __global__ __launch_bounds__(kTile) void k_sched_scatter(const uint8_t* __restri
   list[pos] = (uint32_t)k;
 }
 
-// bounds[s] = first list position of slot s; bounds[3] = n.
+// bounds[s] = first list position of slot s; bounds[kSchedSlots] = n.
 __global__ void k_sched_bounds(const uint32_t* __restrict__ start, int ntiles, int n, uint32_t* __restrict__ bounds) {
   const int s = threadIdx.x;
-  if (s < 3) bounds[s] = start[(size_t)(s * kSchedBuckets) * ntiles];
-  if (s == 3) bounds[3] = (uint32_t)n;
+  if (s < kSchedSlots) bounds[s] = start[(size_t)(s * kSchedBuckets) * ntiles];
+  if (s == kSchedSlots) bounds[kSchedSlots] = (uint32_t)n;
 }
 
 __global__ void k_mark_status(const uint32_t* __restrict__ list, int m, const uint32_t* __restrict__ sorted,
@@ -169,12 +189,16 @@ hipError_t launch_batch_check(const uint32_t* delim, const uint64_t* dend, const
 
 hipError_t launch_schedule(const float* y, int stride, float scale, float shift, float t1, float t2,
                            const uint32_t* sorted, int n, uint8_t* key, uint32_t* hist, uint32_t* sums,
-                           uint32_t* list, uint32_t* bounds, hipStream_t s) {
+                           uint32_t* list, uint32_t* bounds, hipStream_t s, const SchedRadius* radius) {
   if (n <= 0) return hipSuccess;
   const int ntiles = blocks(n, kTile);
   const int m = kSchedKeys * ntiles;
-  hipLaunchKernelGGL(k_sched_hist, dim3(ntiles), dim3(kTile), 0, s, y, stride, scale, shift, t1, t2, sorted, n, key,
-                     hist, ntiles);
+  SchedRadius r;
+  if (radius) r = *radius;
+  hipError_t e0 = launch_fill_u32(bounds + kSchedSlots + 1, 1, 0u, s);
+  if (e0 != hipSuccess) return e0;
+  hipLaunchKernelGGL(k_sched_hist, dim3(ntiles), dim3(kTile), 0, s, y, stride, scale, shift, t1, t2, r, sorted, n, key,
+                     hist, ntiles, bounds + kSchedSlots + 1);
   // exclusive scan in place: hist becomes the start of each (key, tile)
   hipError_t e = launch_scan_u32(hist, m, hist, sums, sums + blocks(m, 1024), s);
   if (e != hipSuccess) return e;

@@ -42,6 +42,22 @@ struct Summary {
 // their faces into a shared arena instead (CellArgs::fpos).
 constexpr int kFaceStage = 64;
 
+// The face arena is split into one region per XCD (workgroups are dealt to
+// the 8 XCDs in turn, so a wave's region is its block number mod 8), each
+// 7/64 of the capacity, and a shared overflow region (the last 1/8): the
+// ranges that neighbouring waves reserve are written through one L2 and
+// leave it as whole lines.  Counters sit one 128-byte line apart: the 8
+// regions', the overflow's, then the count of cells that found no room.
+constexpr int kArenaParts = 8;
+constexpr int kArenaStride = 16;  // u64 words between counters
+constexpr int kArenaWords = (kArenaParts + 2) * kArenaStride;
+__host__ __device__ inline uint64_t arena_part_size(uint64_t cap) { return cap / 64 * 7; }
+// Faces an arena of n cells holds: 64 per cell, at least 2^16 (so that every
+// region of a small set still takes a cell of hundreds of faces).
+__host__ __device__ inline uint64_t arena_capacity(uint64_t n) {
+  return n * kFaceStage > (1ull << 16) ? n * kFaceStage : (1ull << 16);
+}
+
 // Learned search-radius model 27 -> 64 -> 64 -> 2.  Parameter vector layout
 // (FP32 master copy): W1 [32][64] (rows 27..31 are the zero feature pad),
 // W2 [64][64], W3 [64][16] (columns 2..15 unused), b1 [64], b2 [64],
@@ -84,9 +100,11 @@ struct CellArgs {
   int32_t* snbr;           // face arena (fpos set) or per-cell slots [n][kFaceStage]
   double* sarea;
   uint64_t* fpos;          // nullable: cell o's faces start at snbr[fpos[o]], a
-                           // range reserved by one atomic on arena_next (no
-                           // per-cell face limit); null: at o * kFaceStage
-  unsigned long long* arena_next;
+                           // range reserved by one atomic on the arena's
+                           // counters (no per-cell face limit); null: at
+                           // o * kFaceStage
+  unsigned long long* arena_next;  // kArenaWords counters (kArenaParts regions,
+                                   // overflow, cells without room)
   uint64_t arena_cap;      // faces the arena holds
   uint32_t* status;
   tess_cell_stats* stats;  // nullable

@@ -23,7 +23,7 @@ STATUS = {0: "TESS_OK", 1: "TESS_E_INVALID", 2: "TESS_E_NOMEM", 3: "TESS_E_DEVIC
           4: "TESS_E_CAPACITY", 5: "TESS_E_DEGENERATE", 6: "TESS_E_STATE", 7: "TESS_E_NOTFOUND"}
 MODE_PLAIN, MODE_LEARNED = 0, 1
 CELL_CAPACITY, CELL_TABLE, CELL_DEGENERATE, CELL_FALLBACK, CELL_RERUN = 1, 2, 4, 8, 16
-CELL_LONG_TABLE, CELL_SPILL = 32, 64
+CELL_LONG_TABLE, CELL_SPILL, CELL_ARENA = 32, 64, 128
 NOALLOC_MAX_POINTS = 16384
 
 EXPORTS = [
@@ -81,7 +81,8 @@ class Timings(C.Structure):
     _fields_ = [("bin_ms", C.c_double), ("cells_ms", C.c_double), ("compact_ms", C.c_double),
                 ("total_ms", C.c_double), ("table_ms", C.c_double), ("flops", C.c_uint64),
                 ("fallback_cells", C.c_uint64), ("capacity_reruns", C.c_uint64), ("plan_ms", C.c_double),
-                ("table_reruns", C.c_uint64), ("spill_cells", C.c_uint64), ("incident_cuts", C.c_uint64)]
+                ("table_reruns", C.c_uint64), ("spill_cells", C.c_uint64), ("incident_cuts", C.c_uint64),
+                ("planned_long", C.c_uint64)]
 
 
 class BatchConfig(C.Structure):
