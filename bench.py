@@ -381,7 +381,7 @@ def bench_fcc(ctx, w, args):
 
     plain_ms, plain_t = timed(plain_step, max(2, args.steps // 2), 1, ctx)
     b = tess.Batch(ctx, sets, density=w["density"])
-    b.set_config(learned=1)
+    b.set_config(**(fixed_config(args.batch_config) if args.batch_config else dict(learned=1)))
 
     def step():
         b.compute(m)
@@ -764,7 +764,7 @@ def main():
         # volume + face count + status (16 B), 12 B per staged face
         line["roofline"]["alg_hbm_gbs"] = (44 * n + 12 * faces) / (kernel_ms / 1e3) / 1e9
     line.update(extra)
-    if w["kind"] in ("clustered", "uniform"):
+    if w["kind"] in ("clustered", "uniform") and not args.plain_only:
         line["end_to_end"] = end_to_end(ctx, pts, lo, hi, w["density"])
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
