@@ -49,9 +49,14 @@ __device__ __forceinline__ void wave_sync() {
 }
 __device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
 __device__ __forceinline__ int ctz64(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+// Set bits of m below this lane (lane = the lane's index in its wavefront).
 __device__ __forceinline__ int mbcnt(uint64_t m, int lane) {
-  return __popcll(m & ((1ull << lane) - 1ull));
+  (void)lane;
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
+// Bit `lane` of a wave-uniform mask as this lane's predicate: the mask is
+// used as the lane mask it is, with no per-lane shift.
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 __device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 // A value every lane holds (e.g. an LDS broadcast read) as a scalar.
 __device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -343,7 +348,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
       continue;
     }
     const int p = lane + 64 * r;
-    const bool al = (C.PA[r] >> lane) & 1ull;
+    const bool al = lane_bit(C.PA[r]);
     uint32_t c0 = C_IN, c1 = C_IN;
     if (al) {
       c0 = S.vcls[S.he_tgt[2 * p]];
@@ -389,7 +394,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
 #pragma unroll
     for (int r = 0; r < VPL; ++r) {
       const int k = bv + mbcnt(vfree[r], lane);
-      const bool take = ((vfree[r] >> lane) & 1ull) && k < ncross;
+      const bool take = lane_bit(vfree[r]) && k < ncross;
       if (take) S.cs.f.vslot[k] = (uint8_t)(lane + 64 * r);
       vnew[r] = ballot(take);
       bv += popc(vfree[r]);
@@ -398,7 +403,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
       const int k = bp + mbcnt(pfree[r], lane);
-      const bool take = ((pfree[r] >> lane) & 1ull) && k < ncross;
+      const bool take = lane_bit(pfree[r]) && k < ncross;
       if (take) S.cs.f.pslot[k] = (typename Lds<VPL, PPL, FPL>::PairSlot)(lane + 64 * r);
       pnew[r] = ballot(take);
       bp += popc(pfree[r]);
@@ -482,7 +487,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     for (int r = 0; r < PPL; ++r) {
       if (r > 0 && C.PA[r] == 0) continue;
       const int q = lane + 64 * r;
-      if ((C.PA[r] >> lane) & 1ull) {
+      if (lane_bit(C.PA[r])) {
         S.fmark[S.he_face[2 * q]] = 1;
         S.fmark[S.he_face[2 * q + 1]] = 1;
       }
@@ -499,7 +504,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
 #pragma unroll
   for (int r = 0; r < VPL; ++r) {
     const int v = lane + 64 * r;
-    if ((vnew[r] >> lane) & 1ull) {
+    if (lane_bit(vnew[r])) {
       C.rx[r] = S.vx[v];
       C.ry[r] = S.vy[v];
       C.rz[r] = S.vz[v];
@@ -525,7 +530,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
     const int p = lane + 64 * r;
-    const bool al = (C.PA[r] >> lane) & 1ull;
+    const bool al = lane_bit(C.PA[r]);
     bool ent = false, del = false;
     int eh = 0;
     if (al) {
@@ -697,7 +702,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
     const int p = lane + 64 * r;
-    if ((C.PA[r] >> lane) & 1ull) {
+    if (lane_bit(C.PA[r])) {
       S.fmark[S.he_face[2 * p]] = 1;
       S.fmark[S.he_face[2 * p + 1]] = 1;
     }
@@ -709,7 +714,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
 #pragma unroll
   for (int r = 0; r < VPL; ++r) {
     const int v = lane + 64 * r;
-    if ((C.VA[r] >> lane) & 1ull) {
+    if (lane_bit(C.VA[r])) {
       C.rx[r] = S.vx[v];
       C.ry[r] = S.vy[v];
       C.rz[r] = S.vz[v];
@@ -724,7 +729,7 @@ __device__ double cell_r2(const CellState<VPL, PPL, FPL>& C, int lane) {
   double m = 0.0;
 #pragma unroll
   for (int r = 0; r < VPL; ++r)
-    if ((C.VA[r] >> lane) & 1ull) {
+    if (lane_bit(C.VA[r])) {
       double d = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
       m = d > m ? d : m;
     }
@@ -739,13 +744,13 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
 #pragma unroll
   for (int r = 0; r < VPL; ++r) {
     d[r] = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
-    if ((C.VA[r] >> lane) & 1ull) m = d[r] > m ? d[r] : m;
+    if (lane_bit(C.VA[r])) m = d[r] > m ? d[r] : m;
   }
   m = wave_max_uni(m);
   int s = -1;
 #pragma unroll
   for (int r = VPL - 1; r >= 0; --r) {
-    const uint64_t b = ballot(((C.VA[r] >> lane) & 1ull) && d[r] == m);
+    const uint64_t b = ballot(lane_bit(C.VA[r]) && d[r] == m);
     if (b) s = 64 * r + ctz64(b);
   }
   *slot = s;
@@ -825,6 +830,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   double r2 = cell_r2_slot(C, lane, &r2_slot);
 
   uint32_t streamed = 0, classified = 0, ncuts = 0, ninc = 0, status = 0;
+  uint32_t cls_base = 0;  // classifications already in flops
   uint64_t flops = 24 + 40;
   int maxv = 8;
   int e_lastcut = -1;
@@ -970,7 +976,6 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         const uint64_t before = next ? (pv & ((1ull << ctz64(next)) - 1ull)) : pv;
         const int nb = popc(before);
         classified += (uint32_t)nb;
-        flops += (uint64_t)nb * (13ull + 6ull * (uint64_t)nv);
         pv &= ~before;
         if (!next) break;
         const int q = ctz64(next);
@@ -978,7 +983,6 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         const double cnx = uni_d(S.u.cand.n[q][0]), cny = uni_d(S.u.cand.n[q][1]);
         const double cnz = uni_d(S.u.cand.n[q][2]), coff = uni_d(S.u.cand.n[q][3]);
         classified++;
-        flops += 13 + 6ull * (uint64_t)nv;
         // Outside test only; the three-way classification is needed only
         // when the candidate cuts (same arithmetic both times)
         uint64_t OM[VPL];
@@ -998,7 +1002,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         if (!any) continue;
 #pragma unroll
         for (int r = 0; r < VPL; ++r)
-          if ((C.VA[r] >> lane) & 1ull) {
+          if (lane_bit(C.VA[r])) {
             const double sd = dv[r] - coff;
             S.vcls[lane + 64 * r] = sd > kTol ? C_OUT : (sd < -kTol ? C_IN : C_INC);
             S.vdot[lane + 64 * r] = dv[r];
@@ -1023,6 +1027,9 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         }
         ncuts++;
         ninc += newv == 0;  // no Inside-Outside edge: only Incident vertices border the cut
+        // 13 + 6V per classification, V the vertex count it was tested against
+        flops += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
+        cls_base = classified;
         nv = popc_multi(C.VA);
         if (nv > maxv) maxv = nv;
         flops += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
@@ -1034,7 +1041,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           uint64_t big = 0;
 #pragma unroll
           for (int r = 0; r < VPL; ++r) {
-            const bool isnew = ((C.VA[r] & ~kept[r]) >> lane) & 1ull;
+            const bool isnew = lane_bit(C.VA[r] & ~kept[r]);
             const double d = (C.rx[r] * C.rx[r] + C.ry[r] * C.ry[r]) + C.rz[r] * C.rz[r];
             big |= ballot(isnew && d > r2);
           }
@@ -1056,6 +1063,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           pv &= ballot(lane < nkeep && S.u.cand.ent[lane] < e_stop && !(S.u.cand.d2[lane] > thr));
         }
       }
+      flops += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
+      cls_base = classified;
       if (!skip_cells) streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
       // advance the cursor past this chunk's 64 candidates
       if (total <= 64) {
@@ -1118,7 +1127,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 #pragma unroll
     for (int r = 0; r < FPL; ++r) {
       const int f = lane + 64 * r;
-      fal[r] = (C.FA[r] >> lane) & 1ull;
+      fal[r] = lane_bit(C.FA[r]);
       area[r] = 0;
       contrib[r] = 0;
       tagv[r] = 0;
@@ -1216,7 +1225,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         A.exp_v[3 * v] = S.vx[v];
         A.exp_v[3 * v + 1] = S.vy[v];
         A.exp_v[3 * v + 2] = S.vz[v];
-        A.exp_meta[2 + v] = (int)((C.VA[r] >> lane) & 1ull);
+        A.exp_meta[2 + v] = (int)lane_bit(C.VA[r]);
       }
 #pragma unroll
       for (int r = 0; r < FPL; ++r)
@@ -1245,7 +1254,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       double* sv = A.geo_v + (size_t)slot * (3 * L::VC);
 #pragma unroll
       for (int r = 0; r < VPL; ++r) {
-        if ((C.VA[r] >> lane) & 1ull) {
+        if (lane_bit(C.VA[r])) {
           const int q = below + mbcnt(C.VA[r], lane);
           const int v = lane + 64 * r;
           sv[3 * q] = S.vx[v];
