@@ -66,45 +66,36 @@ __device__ __forceinline__ double uni_d(double v) {
   int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-// v of the lane the DPP control names (the lane's own v where it has none
-// or its row is masked off).
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_d(double v) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = (int)(u & 0xffffffffu), hi = (int)(u >> 32);
-  const int slo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
-  const int shi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)shi << 32) | (unsigned)slo));
+// Running maximum of unsigned v over the lanes (inclusive): DPP row shifts
+// inside the rows of 16 (lanes without a source read 0, the identity), then
+// the row maxima broadcast to the later rows.
+__device__ __forceinline__ uint32_t wave_max_scan_u32(uint32_t v) {
+  uint32_t t;
+  t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true); v = t > v ? t : v;  // row_shr:1
+  t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true); v = t > v ? t : v;  // row_shr:2
+  t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true); v = t > v ? t : v;  // row_shr:4
+  t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true); v = t > v ? t : v;  // row_shr:8
+  t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); v = t > v ? t : v;  // row_bcast:15 -> rows 1, 3
+  t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); v = t > v ? t : v;  // row_bcast:31 -> rows 2, 3
+  return v;
 }
-// Maximum over the wavefront (max is order-free), scalar: DPP row shifts
-// inside the rows of 16, the row maxima broadcast to the later rows, lane 63.
+// Maximum of unsigned v over the wavefront, as a scalar.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_max_scan_u32(v), 63);
+}
+// Maximum over the wavefront of non-negative doubles (no -0, no NaN), as a
+// scalar.  Such doubles order as their bit patterns do as unsigned
+// integers: the largest high word, then the largest low word among the
+// lanes that hold it.
 __device__ __forceinline__ double wave_max_uni(double v) {
-  double t;
-  t = dpp_d<0x111, 0xf>(v); v = t > v ? t : v;  // row_shr:1
-  t = dpp_d<0x112, 0xf>(v); v = t > v ? t : v;  // row_shr:2
-  t = dpp_d<0x114, 0xf>(v); v = t > v ? t : v;  // row_shr:4
-  t = dpp_d<0x118, 0xf>(v); v = t > v ? t : v;  // row_shr:8
-  t = dpp_d<0x142, 0xa>(v); v = t > v ? t : v;  // row_bcast:15 -> rows 1, 3
-  t = dpp_d<0x143, 0xc>(v); v = t > v ? t : v;  // row_bcast:31 -> rows 2, 3
   const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), 63);
-  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  const uint32_t hi = (uint32_t)(u >> 32), lo = (uint32_t)(u & 0xffffffffu);
+  const uint32_t mh = wave_max_u32(hi);
+  const uint32_t ml = wave_max_u32(hi == mh ? lo : 0u);
+  return __longlong_as_double((long long)(((unsigned long long)mh << 32) | ml));
 }
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-// Running maximum over the lanes (inclusive), DPP as wave_incl_scan; lanes
-// without a source contribute -1 (values are >= -1).
-__device__ __forceinline__ int wave_max_scan(int v) {
-  int t;
-  t = __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:1
-  t = __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:2
-  t = __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:4
-  t = __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false); v = t > v ? t : v;  // row_shr:8
-  t = __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false); v = t > v ? t : v;  // row_bcast:15
-  t = __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false); v = t > v ? t : v;  // row_bcast:31
   return v;
 }
 // Inclusive prefix sum over the wavefront with DPP row shifts inside each
@@ -233,6 +224,9 @@ __device__ __forceinline__ uint64_t arena_reserve(unsigned long long* ctr, uint6
 
 // A chunk with at least this many staged candidates is pre-tested lane-parallel.
 constexpr int kPreMin = 16;
+// A chunk looks at up to 64 * kWin stream positions (kWin per lane) and
+// stages the first 64 kept candidates among them.
+constexpr int kWin = 2;
 
 __device__ __forceinline__ bool lex_less(double ax, double ay, double az, double bx, double by, double bz) {
   if (ax != bx) return ax < bx;
@@ -256,21 +250,22 @@ struct Lds {
   // a pair slot number: one byte while there are at most 256 pairs
   using PairSlot = typename std::conditional<(PC > 256), uint16_t, uint8_t>::type;
   double vx[VC], vy[VC], vz[VC], vdot[VC];
-  // kept candidates of the current chunk (in stream order); the output
-  // phase reuses the space for per-face volume terms
+  // kept candidates of the current chunk (in stream order); before them,
+  // the window's entry heads; the output phase reuses the space for
+  // per-face volume terms
   union {
     struct {
-      double n[64][4];  // nx, ny, nz, offset
+      double n[64][4];  // nx, ny, nz, offset (px, py, pz while staging)
       double d2[64];
       int32_t tag[64];
       int32_t ent[64];
     } cand;
+    uint8_t head[64 * kWin];  // the entry whose candidates start at window position p (0xff: none)
     double fcontrib[FC > 64 ? FC : 64];
   } u;
   int32_t f_tag[FC];
   uint32_t bstart[64];
   uint32_t bpref[64];
-  uint8_t head[64];  // streaming: the entry whose candidates start at chunk position p (0xff: none)
   uint16_t he_next[HC];
   uint16_t f_start[FC];
   uint16_t f_seq[FC];
@@ -897,52 +892,97 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       const int pre = incl - c;
       S.bpref[lane] = (uint32_t)pre;
       S.bstart[lane] = bst + (uint32_t)(lane == ce ? co : 0);
-      // entry of chunk position k: the last non-empty entry starting at or
-      // before k (heads by position, then a running maximum)
-      S.head[lane] = 0xff;
+      // The window: stream positions [0, min(total, 64 kWin)), lane + 64 j
+      // in slice j.  Entry of position k: the last non-empty entry starting
+      // at or before k (heads by position, then a running maximum carried
+      // across the slices).
+      const int nsl = total < 64 * kWin ? (total + 63) >> 6 : kWin;
+#pragma unroll
+      for (int j = 0; j < kWin; ++j) S.u.head[lane + 64 * j] = 0xff;
       wave_sync();
-      if (c > 0 && pre < 64) S.head[pre] = (uint8_t)lane;
+      if (c > 0 && pre < 64 * kWin) S.u.head[pre] = (uint8_t)lane;
       wave_sync();
-      const int hk = S.head[lane];
-      const int ek = wave_max_scan(hk == 0xff ? -1 : hk);
-      const int k = lane;
-      bool valid = false;
-      uint32_t jidx = 0;
-      double px = 0, py = 0, pz = 0, d2 = 0;
-      if (k < total) {
-        const uint32_t s = S.bstart[ek] + (uint32_t)(k - (int)S.bpref[ek]);
-        jidx = A.sorted[s];
-        valid = (jidx != self) && (A.target_group < 0 || A.groups[jidx] == A.target_group);
-        px = A.spx[s] - sx;
-        py = A.spy[s] - sy;
-        pz = A.spz[s] - sz;
-        d2 = (px * px + py * py) + pz * pz;
+      int ekw[kWin];
+      {
+        int carry = -1;
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+          ekw[j] = 0;
+          if (j >= nsl) continue;
+          const int hk = S.u.head[lane + 64 * j];
+          int e = (int)wave_max_scan_u32(hk == 0xff ? 0u : (uint32_t)hk + 1u) - 1;
+          e = e > carry ? e : carry;
+          ekw[j] = e;
+          carry = rl_i(e, 63);
+        }
       }
-      const int eg = ebase + ek;
-      const bool keep = valid && eg < e_stop && !(d2 > stop_thr(r2));
-      double nx = 0, ny = 0, nz = 0, off = 0;
-      if (keep) {  // halfway_from_origin_to, vector3.rs:223-239
-        const double mag = sqrt(d2);
+      wave_sync();  // the heads share space with the staged candidates
+      // Stage the kept candidates in stream order, up to 64: the chunk ends
+      // after the 64th (positions past it are streamed again by the next
+      // chunk).  The sequential pass reads them back as wave-uniform values.
+      int consumed = total < 64 * kWin ? total : 64 * kWin;
+      int nkeep = 0;
+      uint64_t vmw[kWin];  // valid positions of each slice (group-filter streamed count)
+#pragma unroll
+      for (int j = 0; j < kWin; ++j) {
+        vmw[j] = 0;
+        if (64 * j >= consumed) continue;
+        const int k = lane + 64 * j;
+        const int ek = ekw[j];
+        bool valid = false;
+        uint32_t jidx = 0;
+        double px = 0, py = 0, pz = 0, d2 = 0;
+        if (k < consumed) {
+          const uint32_t s = S.bstart[ek] + (uint32_t)(k - (int)S.bpref[ek]);
+          jidx = A.sorted[s];
+          valid = (jidx != self) && (A.target_group < 0 || A.groups[jidx] == A.target_group);
+          px = A.spx[s] - sx;
+          py = A.spy[s] - sy;
+          pz = A.spz[s] - sz;
+          d2 = (px * px + py * py) + pz * pz;
+        }
+        // every position of the window lies in an entry before the stop one
+        const bool keep = valid && !(d2 > stop_thr(r2));
+        const uint64_t km = ballot(keep);
+        const int q = nkeep + mbcnt(km, lane);
+        if (keep && q < 64) {
+          S.u.cand.n[q][0] = px;
+          S.u.cand.n[q][1] = py;
+          S.u.cand.n[q][2] = pz;
+          S.u.cand.d2[q] = d2;
+          S.u.cand.tag[q] = (int32_t)jidx;
+          S.u.cand.ent[q] = ebase + ek;
+        }
+        vmw[j] = ballot(valid);
+        const int kc = popc(km);
+        if (nkeep + kc > 64) {
+          consumed = 64 * j + select_bit(km, 63 - nkeep) + 1;  // just past the 64th
+          nkeep = 64;
+        } else {
+          nkeep += kc;
+          if (nkeep == 64 && 64 * (j + 1) < consumed) consumed = 64 * (j + 1);
+        }
+      }
+      wave_sync();
+      // bisector planes, lane = staged candidate (halfway_from_origin_to,
+      // vector3.rs:223-239)
+      double cnx0 = 0, cny0 = 0, cnz0 = 0, coff0 = 0, cd2 = 0;
+      int cent = 0;
+      if (lane < nkeep) {
+        const double px = S.u.cand.n[lane][0], py = S.u.cand.n[lane][1], pz = S.u.cand.n[lane][2];
+        cd2 = S.u.cand.d2[lane];
+        cent = S.u.cand.ent[lane];
+        const double mag = sqrt(cd2);
         const double inv = 1.0 / mag;
-        nx = px * inv;
-        ny = py * inv;
-        nz = pz * inv;
-        off = (nx * (px * 0.5) + ny * (py * 0.5)) + nz * (pz * 0.5);
+        cnx0 = px * inv;
+        cny0 = py * inv;
+        cnz0 = pz * inv;
+        coff0 = (cnx0 * (px * 0.5) + cny0 * (py * 0.5)) + cnz0 * (pz * 0.5);
+        S.u.cand.n[lane][0] = cnx0;
+        S.u.cand.n[lane][1] = cny0;
+        S.u.cand.n[lane][2] = cnz0;
+        S.u.cand.n[lane][3] = coff0;
       }
-      // stage the kept candidates in stream order; the sequential pass reads
-      // them back as wave-uniform values
-      const uint64_t km = ballot(keep);
-      if (keep) {
-        const int q = mbcnt(km, lane);
-        S.u.cand.n[q][0] = nx;
-        S.u.cand.n[q][1] = ny;
-        S.u.cand.n[q][2] = nz;
-        S.u.cand.n[q][3] = off;
-        S.u.cand.d2[q] = d2;
-        S.u.cand.tag[q] = (int32_t)jidx;
-        S.u.cand.ent[q] = eg;
-      }
-      const int nkeep = popc(km);
       wave_sync();
       const uint64_t all_kept = nkeep >= 64 ? ~0ull : ((1ull << nkeep) - 1ull);
       // Pre-test, lane = candidate: which staged candidates cut the cell as it
@@ -954,16 +994,14 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       if (nkeep >= kPreMin) {
         bool cut = false;
         if (lane < nkeep) {
-          const double nx = S.u.cand.n[lane][0], ny = S.u.cand.n[lane][1];
-          const double nz = S.u.cand.n[lane][2], off = S.u.cand.n[lane][3];
 #pragma unroll
           for (int r = 0; r < VPL; ++r) {
             uint64_t m = C.VA[r];
             while (m) {
               const int v = 64 * r + ctz64(m);
               m &= m - 1ull;
-              const double d = (nx * S.vx[v] + ny * S.vy[v]) + nz * S.vz[v];
-              cut = cut || (d - off > kTol);
+              const double d = (cnx0 * S.vx[v] + cny0 * S.vy[v]) + cnz0 * S.vz[v];
+              cut = cut || (d - coff0 > kTol);
             }
           }
         }
@@ -1060,22 +1098,34 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
             stop_here = true;
           }
           // the smaller cell reaches fewer of the pending candidates
-          pv &= ballot(lane < nkeep && S.u.cand.ent[lane] < e_stop && !(S.u.cand.d2[lane] > thr));
+          pv &= ballot(lane < nkeep && cent < e_stop && !(cd2 > thr));
         }
       }
       flops += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
       cls_base = classified;
-      if (!skip_cells) streamed += (uint32_t)popc(ballot(valid && eg < e_stop));
-      // advance the cursor past this chunk's 64 candidates
-      if (total <= 64) {
+      if (!skip_cells) {
+        // the valid positions consumed that lie before the stop entry (a
+        // prefix of the window: positions follow the entries)
+        const int es = e_stop - ebase;
+        const int pe = es >= 64 ? consumed : rl_i(pre, es);
+        const int lim_k = pe < consumed ? pe : consumed;
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+          const int lk = lim_k - 64 * j;
+          if (lk <= 0) break;
+          streamed += (uint32_t)popc(lk >= 64 ? vmw[j] : (vmw[j] & ((1ull << lk) - 1ull)));
+        }
+      }
+      // advance the cursor past the positions this chunk consumed
+      if (consumed >= total) {
         ce = 64;
         co = 0;
       } else {
-        // the entry holding position 64: the last non-empty one starting at
-        // or before it
-        const uint64_t hm = ballot(c > 0 && pre <= 64);
+        // the entry holding position `consumed`: the last non-empty one
+        // starting at or before it
+        const uint64_t hm = ballot(c > 0 && pre <= consumed);
         const int lo = 63 - __clzll((long long)hm);
-        co = 64 - rl_i(pre, lo) + (lo == ce ? co : 0);
+        co = consumed - rl_i(pre, lo) + (lo == ce ? co : 0);
         ce = lo;
       }
       wave_sync();
