@@ -54,18 +54,23 @@ __device__ __forceinline__ int mbcnt(uint64_t m, int lane) {
   (void)lane;
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
+// Keeps v a per-lane value the compiler takes as it is: a ballot of a test
+// on it stays one vector compare instead of a scalar re-derivation from the
+// masks v came from.
+__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
 // Bit `lane` of a wave-uniform mask as this lane's predicate: the mask is
 // used as the lane mask it is, with no per-lane shift.
 __device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 __device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-// A value every lane holds (e.g. an LDS broadcast read) as a scalar.
-__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double uni_d(double v) {
-  unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
-  int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+// Lane l's v as a scalar.
+__device__ __forceinline__ double rl_d(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+// A value every lane holds (e.g. an LDS broadcast read) as a scalar.
+__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // Running maximum of unsigned v over the lanes (inclusive): DPP row shifts
 // inside the rows of 16 (lanes without a source read 0, the identity), then
 // the row maxima broadcast to the later rows.
@@ -249,14 +254,15 @@ struct Lds {
   static constexpr int VC = 64 * VPL, PC = 64 * PPL, HC = 2 * PC, FC = 64 * FPL;
   // a pair slot number: one byte while there are at most 256 pairs
   using PairSlot = typename std::conditional<(PC > 256), uint16_t, uint8_t>::type;
+  // a half-edge number: one byte while there are at most 256 half-edges
+  using HalfEdge = typename std::conditional<(HC > 256), uint16_t, uint8_t>::type;
   double vx[VC], vy[VC], vz[VC], vdot[VC];
   // kept candidates of the current chunk (in stream order); before them,
   // the window's entry heads; the output phase reuses the space for
   // per-face volume terms
   union {
     struct {
-      double n[64][4];  // nx, ny, nz, offset (px, py, pz while staging)
-      double d2[64];
+      double p[64][3];  // candidate - site
       int32_t tag[64];
       int32_t ent[64];
     } cand;
@@ -266,7 +272,7 @@ struct Lds {
   int32_t f_tag[FC];
   uint32_t bstart[64];
   uint32_t bpref[64];
-  uint16_t he_next[HC];
+  HalfEdge he_next[HC];
   uint16_t f_start[FC];
   uint16_t f_seq[FC];
   uint16_t fentry[FC];  // fast clip: the Outside-run entry half-edge of each face
@@ -388,7 +394,8 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     int bv = 0;
 #pragma unroll
     for (int r = 0; r < VPL; ++r) {
-      const int k = bv + mbcnt(vfree[r], lane);
+      int k = bv + mbcnt(vfree[r], lane);
+      opaque(k);
       const bool take = lane_bit(vfree[r]) && k < ncross;
       if (take) S.cs.f.vslot[k] = (uint8_t)(lane + 64 * r);
       vnew[r] = ballot(take);
@@ -397,7 +404,8 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     int bp = 0;
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
-      const int k = bp + mbcnt(pfree[r], lane);
+      int k = bp + mbcnt(pfree[r], lane);
+      opaque(k);
       const bool take = lane_bit(pfree[r]) && k < ncross;
       if (take) S.cs.f.pslot[k] = (typename Lds<VPL, PPL, FPL>::PairSlot)(lane + 64 * r);
       pnew[r] = ballot(take);
@@ -946,10 +954,9 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         const uint64_t km = ballot(keep);
         const int q = nkeep + mbcnt(km, lane);
         if (keep && q < 64) {
-          S.u.cand.n[q][0] = px;
-          S.u.cand.n[q][1] = py;
-          S.u.cand.n[q][2] = pz;
-          S.u.cand.d2[q] = d2;
+          S.u.cand.p[q][0] = px;
+          S.u.cand.p[q][1] = py;
+          S.u.cand.p[q][2] = pz;
           S.u.cand.tag[q] = (int32_t)jidx;
           S.u.cand.ent[q] = ebase + ek;
         }
@@ -966,22 +973,18 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       wave_sync();
       // bisector planes, lane = staged candidate (halfway_from_origin_to,
       // vector3.rs:223-239)
-      double cnx0 = 0, cny0 = 0, cnz0 = 0, coff0 = 0, cd2 = 0;
-      int cent = 0;
+      // (kept in this lane's registers: the sequential pass reads candidate
+      // q's plane from lane q)
+      double cnx0 = 0, cny0 = 0, cnz0 = 0, coff0 = 0;
       if (lane < nkeep) {
-        const double px = S.u.cand.n[lane][0], py = S.u.cand.n[lane][1], pz = S.u.cand.n[lane][2];
-        cd2 = S.u.cand.d2[lane];
-        cent = S.u.cand.ent[lane];
+        const double px = S.u.cand.p[lane][0], py = S.u.cand.p[lane][1], pz = S.u.cand.p[lane][2];
+        const double cd2 = (px * px + py * py) + pz * pz;  // the staging's d2, bit for bit
         const double mag = sqrt(cd2);
         const double inv = 1.0 / mag;
         cnx0 = px * inv;
         cny0 = py * inv;
         cnz0 = pz * inv;
         coff0 = (cnx0 * (px * 0.5) + cny0 * (py * 0.5)) + cnz0 * (pz * 0.5);
-        S.u.cand.n[lane][0] = cnx0;
-        S.u.cand.n[lane][1] = cny0;
-        S.u.cand.n[lane][2] = cnz0;
-        S.u.cand.n[lane][3] = coff0;
       }
       wave_sync();
       const uint64_t all_kept = nkeep >= 64 ? ~0ull : ((1ull << nkeep) - 1ull);
@@ -1018,8 +1021,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         if (!next) break;
         const int q = ctz64(next);
         pv &= pv - 1ull;
-        const double cnx = uni_d(S.u.cand.n[q][0]), cny = uni_d(S.u.cand.n[q][1]);
-        const double cnz = uni_d(S.u.cand.n[q][2]), coff = uni_d(S.u.cand.n[q][3]);
+        const double cnx = rl_d(cnx0, q), cny = rl_d(cny0, q);
+        const double cnz = rl_d(cnz0, q), coff = rl_d(coff0, q);
         classified++;
         // Outside test only; the three-way classification is needed only
         // when the candidate cuts (same arithmetic both times)
@@ -1098,7 +1101,12 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
             stop_here = true;
           }
           // the smaller cell reaches fewer of the pending candidates
-          pv &= ballot(lane < nkeep && cent < e_stop && !(cd2 > thr));
+          bool reach = false;
+          if (lane < nkeep && S.u.cand.ent[lane] < e_stop) {
+            const double px = S.u.cand.p[lane][0], py = S.u.cand.p[lane][1], pz = S.u.cand.p[lane][2];
+            reach = !(((px * px + py * py) + pz * pz) > thr);
+          }
+          pv &= ballot(reach);
         }
       }
       flops += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
@@ -1392,9 +1400,14 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   return status;
 }
 
+// Wavefronts per SIMD the register allocation aims at: six for the
+// capacity-64 kernels (6,784 B of LDS each, 24 per CU), five for the larger
+// ones (their LDS caps them lower).
+template <int VPL> constexpr int kOcc = VPL == 1 ? 6 : 5;
+
 // One cell per wavefront over a set's grid positions (all, or a work list).
 template <int VPL, int PPL, int FPL, int GEO = 0>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells(CellArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc<VPL>))) void k_cells(CellArgs A) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
   const int w = A.xcd_chunk ? (int)xcd_map(blockIdx.x, (uint32_t)A.xcd_chunk) : (int)blockIdx.x;
@@ -1434,7 +1447,7 @@ __device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<
 // the hardware dispatcher hands out the items in list order as wavefront
 // slots free up, which is the work queue of the pass.
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cells_items(BatchArgs B, uint32_t n) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc<VPL>))) void k_cells_items(BatchArgs B, uint32_t n) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const uint32_t i = B.xcd_chunk ? xcd_map(blockIdx.x, (uint32_t)B.xcd_chunk) : blockIdx.x;
   if (i >= n) return;
@@ -1479,13 +1492,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 __device__ __forceinline__ void ws_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ double rl_d(double v, int l) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
-  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 struct SpillWs {
