@@ -256,7 +256,7 @@ struct Lds {
   using PairSlot = typename std::conditional<(PC > 256), uint16_t, uint8_t>::type;
   // a half-edge number: one byte while there are at most 256 half-edges
   using HalfEdge = typename std::conditional<(HC > 256), uint16_t, uint8_t>::type;
-  double vx[VC], vy[VC], vz[VC], vdot[VC];
+  double vx[VC], vy[VC], vz[VC];
   // kept candidates of the current chunk (in stream order); before them,
   // the window's entry heads; the output phase reuses the space for
   // per-face volume terms
@@ -270,16 +270,18 @@ struct Lds {
     double fcontrib[FC > 64 ? FC : 64];
   } u;
   int32_t f_tag[FC];
-  uint32_t bstart[64];
-  uint32_t bpref[64];
   HalfEdge he_next[HC];
   uint16_t f_start[FC];
   uint16_t f_seq[FC];
   uint16_t fentry[FC];  // fast clip: the Outside-run entry half-edge of each face
   uint16_t fexit[FC];   // ... and its exit half-edge
-  // per-cut scratch of the general clip and of the fast clip (never live
-  // at the same time)
+  // per-cut scratch of the general clip and of the fast clip, and the
+  // staging's entry table (never live at the same time)
   union {
+    struct {
+      uint32_t bstart[64];  // first candidate position of each entry
+      uint32_t bpref[64];   // its first chunk position
+    } st;
     struct {
       uint16_t exit_of[VC];
       uint16_t run_a[64];
@@ -309,6 +311,13 @@ struct CellState {
   double rx[VPL], ry[VPL], rz[VPL];  // this lane's vertex slots (register copy)
 };
 
+// n.v of vertex v: the sequential test's arithmetic on the same operands,
+// so bit for bit its value.
+template <int VPL, int PPL, int FPL>
+__device__ __forceinline__ double ndot(const Lds<VPL, PPL, FPL>& S, int v, double nx, double ny, double nz) {
+  return (nx * S.vx[v] + ny * S.vy[v]) + nz * S.vz[v];
+}
+
 // Rank of pair p among the set bits of m (bits below p).
 template <int W>
 __device__ __forceinline__ int rank_multi(const uint64_t (&m)[W], int p) {
@@ -334,8 +343,8 @@ __device__ __forceinline__ int rank_multi(const uint64_t (&m)[W], int p) {
 // one.
 template <int VPL, int PPL, int FPL>
 __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C,
-                                         const uint64_t (&OM)[VPL], double off, int32_t tag, int lane,
-                                         int* newv) {
+                                         const uint64_t (&OM)[VPL], double pnx, double pny, double pnz, double off,
+                                         int32_t tag, int lane, int* newv) {
   // 1. scan pairs; crossing pairs get a rank (prefix of the crossing ballot)
   //    and are listed in rank order with their targets and faces
   uint64_t CM[PPL], PD[PPL];
@@ -425,7 +434,7 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     const int o = S.he_tgt[hin], i = S.he_tgt[hin ^ 1];
     fin = S.he_face[hin];
     fout = S.he_face[hin ^ 1];
-    const double dO = S.vdot[o], dI = S.vdot[i];
+    const double dO = ndot(S, o, pnx, pny, pnz), dI = ndot(S, i, pnx, pny, pnz);
     const double t = (off - dO) / (dI - dO);
     const double ox = S.vx[o], oy = S.vy[o], oz = S.vz[o];
     nx = ox + (S.vx[i] - ox) * t;
@@ -518,10 +527,10 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
 }
 
 // Apply one bisector that puts at least one vertex Outside.  OM: Outside
-// vertices; S.vcls / S.vdot hold every live vertex's class and n.v.
+// vertices; S.vcls holds every live vertex's class (n.v is recomputed).
 template <int VPL, int PPL, int FPL>
 __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, FPL>& C, const uint64_t (&OM)[VPL],
-                    double off, int32_t tag, int lane, int* newv) {
+                    double pnx, double pny, double pnz, double off, int32_t tag, int lane, int* newv) {
   constexpr int HC = Lds<VPL, PPL, FPL>::HC;
 #pragma unroll
   for (int r = 0; r < VPL; ++r) S.cs.g.exit_of[lane + 64 * r] = 0xffff;
@@ -630,7 +639,7 @@ __device__ __forceinline__ int clip(Lds<VPL, PPL, FPL>& S, CellState<VPL, PPL, F
   // 4. exit points: intersection(outside, inside), vector3.rs:213-219
   double X0 = 0, X1 = 0, X2 = 0;
   if (needX) {
-    const double dO = S.vdot[Ok], dQ = S.vdot[Q];
+    const double dO = ndot(S, Ok, pnx, pny, pnz), dQ = ndot(S, Q, pnx, pny, pnz);
     const double t = (off - dO) / (dQ - dO);
     const double ox = S.vx[Ok], oy = S.vy[Ok], oz = S.vz[Ok];
     X0 = ox + (S.vx[Q] - ox) * t;
@@ -898,8 +907,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       const int total = rl_i(incl, 63);
       if (total == 0) break;
       const int pre = incl - c;
-      S.bpref[lane] = (uint32_t)pre;
-      S.bstart[lane] = bst + (uint32_t)(lane == ce ? co : 0);
+      S.cs.st.bpref[lane] = (uint32_t)pre;
+      S.cs.st.bstart[lane] = bst + (uint32_t)(lane == ce ? co : 0);
       // The window: stream positions [0, min(total, 64 kWin)), lane + 64 j
       // in slice j.  Entry of position k: the last non-empty entry starting
       // at or before k (heads by position, then a running maximum carried
@@ -941,7 +950,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         uint32_t jidx = 0;
         double px = 0, py = 0, pz = 0, d2 = 0;
         if (k < consumed) {
-          const uint32_t s = S.bstart[ek] + (uint32_t)(k - (int)S.bpref[ek]);
+          const uint32_t s = S.cs.st.bstart[ek] + (uint32_t)(k - (int)S.cs.st.bpref[ek]);
           jidx = A.sorted[s];
           valid = (jidx != self) && (A.target_group < 0 || A.groups[jidx] == A.target_group);
           px = A.spx[s] - sx;
@@ -1046,7 +1055,6 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           if (lane_bit(C.VA[r])) {
             const double sd = dv[r] - coff;
             S.vcls[lane + 64 * r] = sd > kTol ? C_OUT : (sd < -kTol ? C_IN : C_INC);
-            S.vdot[lane + 64 * r] = dv[r];
           }
         wave_sync();
         int newv = 0;
@@ -1059,8 +1067,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           kept[r] = C.VA[r] & ~OM[r];
           if ((r2_slot >> 6) == r) max_cut = (OM[r] >> (r2_slot & 63)) & 1ull;
         }
-        int rc = clip_fast(S, C, OM, coff, ctag, lane, &newv);
-        if (rc == CUT_GENERAL) rc = clip(S, C, OM, coff, ctag, lane, &newv);
+        int rc = clip_fast(S, C, OM, cnx, cny, cnz, coff, ctag, lane, &newv);
+        if (rc == CUT_GENERAL) rc = clip(S, C, OM, cnx, cny, cnz, coff, ctag, lane, &newv);
         if (rc != CUT_OK) {
           status |= (rc == CUT_CAPACITY) ? TESS_CELL_CAPACITY : TESS_CELL_DEGENERATE;
           done = true;
@@ -1325,19 +1333,19 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       wave_sync();
 #pragma unroll
       for (int r = 0; r < FPL; ++r)
-        if (fal[r]) S.bpref[rt[r]] = (uint32_t)ringk[r];
+        if (fal[r]) S.cs.st.bpref[rt[r]] = (uint32_t)ringk[r];
       wave_sync();
-      const int kk = lane < nfaces ? (int)S.bpref[lane] : 0;
+      const int kk = lane < nfaces ? (int)S.cs.st.bpref[lane] : 0;
       const int incl = wave_incl_scan(kk, lane);
       const int ex = incl - kk;
       if (lane < nfaces) A.geo_roff[(size_t)slot * kFaceStage + lane] = (uint16_t)ex;
-      S.bstart[lane] = (uint32_t)ex;
+      S.cs.st.bstart[lane] = (uint32_t)ex;
       wave_sync();
       uint16_t* sr = A.geo_ring + (size_t)slot * L::HC;
 #pragma unroll
       for (int r = 0; r < FPL; ++r)
         if (fal[r]) {
-          const int pos = (int)S.bstart[rt[r]];
+          const int pos = (int)S.cs.st.bstart[rt[r]];
           int h = hanc[r], j = 0;
           do {
             const int v = S.he_tgt[h];
