@@ -359,11 +359,11 @@ __device__ __forceinline__ int clip_fast(Lds<VPL, PPL, FPL>& S, CellState<VPL, P
     }
     const int p = lane + 64 * r;
     const bool al = lane_bit(C.PA[r]);
-    uint32_t c0 = C_IN, c1 = C_IN;
-    if (al) {
-      c0 = S.vcls[S.he_tgt[2 * p]];
-      c1 = S.vcls[S.he_tgt[2 * p + 1]];
-    }
+    // read on every lane (a dead pair's stale target, kept in range, is
+    // masked off below): no divergent branch around the reads
+    constexpr int VC = Lds<VPL, PPL, FPL>::VC;
+    const uint32_t c0 = S.vcls[S.he_tgt[2 * p] & (VC - 1)];
+    const uint32_t c1 = S.vcls[S.he_tgt[2 * p + 1] & (VC - 1)];
     const bool o0 = c0 == C_OUT, o1 = c1 == C_OUT;
     const bool cross = al && ((o0 && c1 == C_IN) || (o1 && c0 == C_IN));
     slow = slow || (al && ((o0 && c1 == C_INC) || (o1 && c0 == C_INC)));
@@ -1023,7 +1023,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       int nv = popc_multi(C.VA);
       for (;;) {
         const uint64_t next = pv & mc;
-        const uint64_t before = next ? (pv & ((1ull << ctz64(next)) - 1ull)) : pv;
+        const uint64_t before = pv & (next - 1ull) & ~next;  // pending below the next cutter (all if none)
         const int nb = popc(before);
         classified += (uint32_t)nb;
         pv &= ~before;
