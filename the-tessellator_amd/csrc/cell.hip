@@ -71,6 +71,12 @@ __device__ __forceinline__ double rl_d(double v, int l) {
 }
 // A value every lane holds (e.g. an LDS broadcast read) as a scalar.
 __device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni_d(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 // Running maximum of unsigned v over the lanes (inclusive): DPP row shifts
 // inside the rows of 16 (lanes without a source read 0, the identity), then
 // the row maxima broadcast to the later rows.
@@ -783,15 +789,18 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   // a site of the set excludes itself by index (interface.rs:299-305); an
   // outside site (interface.rs:211-232) excludes nothing
   const bool ext = A.ext != nullptr;
-  const uint32_t self = ext ? 0xffffffffu : A.sorted[gpos];
-  const double sx = ext ? A.ext[3 * gpos] : A.spx[gpos];
-  const double sy = ext ? A.ext[3 * gpos + 1] : A.spy[gpos];
-  const double sz = ext ? A.ext[3 * gpos + 2] : A.spz[gpos];
-  const GridDev G = *A.grid;
+  const uint32_t self = (uint32_t)uni_i((int)(ext ? 0xffffffffu : A.sorted[gpos]));
+  // the site and the grid are wave-uniform: scalar registers (the grid read
+  // through the constant address space), not vector ones
+  const double sx = uni_d(ext ? A.ext[3 * gpos] : A.spx[gpos]);
+  const double sy = uni_d(ext ? A.ext[3 * gpos + 1] : A.spy[gpos]);
+  const double sz = uni_d(ext ? A.ext[3 * gpos + 2] : A.spz[gpos]);
+  typedef const __attribute__((address_space(4))) GridDev ConstGrid;
+  const GridDev& G = *(const GridDev*)(ConstGrid*)A.grid;
   const int cpd = G.cpd;
-  const int ix = axis_index(sx, G.xmin, G.xmax, G.ix, cpd);
-  const int iy = axis_index(sy, G.ymin, G.ymax, G.iy, cpd);
-  const int iz = axis_index(sz, G.zmin, G.zmax, G.iz, cpd);
+  const int ix = uni_i(axis_index(sx, G.xmin, G.xmax, G.ix, cpd));
+  const int iy = uni_i(axis_index(sy, G.ymin, G.ymax, G.iy, cpd));
+  const int iz = uni_i(axis_index(sz, G.zmin, G.zmax, G.iz, cpd));
 
   // ---- container cube translated by -site (polyhedron.rs:226-392, :859-867)
   {
@@ -1412,6 +1421,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 // capacity-64 kernels (6,784 B of LDS each, 24 per CU), five for the larger
 // ones (their LDS caps them lower).
 template <int VPL> constexpr int kOcc = VPL == 1 ? 6 : 5;
+template <int VPL> constexpr int kOccItems = VPL == 1 ? 6 : 5;
 
 // One cell per wavefront over a set's grid positions (all, or a work list).
 template <int VPL, int PPL, int FPL, int GEO = 0>
@@ -1455,7 +1465,7 @@ __device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<
 // the hardware dispatcher hands out the items in list order as wavefront
 // slots free up, which is the work queue of the pass.
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc<VPL>))) void k_cells_items(BatchArgs B, uint32_t n) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOccItems<VPL>))) void k_cells_items(BatchArgs B, uint32_t n) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const uint32_t i = B.xcd_chunk ? xcd_map(blockIdx.x, (uint32_t)B.xcd_chunk) : blockIdx.x;
   if (i >= n) return;
@@ -1740,15 +1750,18 @@ __device__ double spill_r2(const double* R, int ne2, int lane) {
 __device__ void spill_cell(const CellArgs& A, const int gpos, const SpillArgs& P, SpillWs& W, uint64_t item,
                            int lane) {
   const bool ext = A.ext != nullptr;
-  const uint32_t self = ext ? 0xffffffffu : A.sorted[gpos];
-  const double sx = ext ? A.ext[3 * gpos] : A.spx[gpos];
-  const double sy = ext ? A.ext[3 * gpos + 1] : A.spy[gpos];
-  const double sz = ext ? A.ext[3 * gpos + 2] : A.spz[gpos];
-  const GridDev G = *A.grid;
+  const uint32_t self = (uint32_t)uni_i((int)(ext ? 0xffffffffu : A.sorted[gpos]));
+  // the site and the grid are wave-uniform: scalar registers (the grid read
+  // through the constant address space), not vector ones
+  const double sx = uni_d(ext ? A.ext[3 * gpos] : A.spx[gpos]);
+  const double sy = uni_d(ext ? A.ext[3 * gpos + 1] : A.spy[gpos]);
+  const double sz = uni_d(ext ? A.ext[3 * gpos + 2] : A.spz[gpos]);
+  typedef const __attribute__((address_space(4))) GridDev ConstGrid;
+  const GridDev& G = *(const GridDev*)(ConstGrid*)A.grid;
   const int cpd = G.cpd;
-  const int ix = axis_index(sx, G.xmin, G.xmax, G.ix, cpd);
-  const int iy = axis_index(sy, G.ymin, G.ymax, G.iy, cpd);
-  const int iz = axis_index(sz, G.zmin, G.zmax, G.iz, cpd);
+  const int ix = uni_i(axis_index(sx, G.xmin, G.xmax, G.ix, cpd));
+  const int iy = uni_i(axis_index(sy, G.ymin, G.ymax, G.iy, cpd));
+  const int iz = uni_i(axis_index(sz, G.zmin, G.zmax, G.iz, cpd));
   // container cube translated by -site: face f's ring from its start half-edge
   {
     const double x0 = A.lo[0] + (-sx), x1 = A.hi[0] + (-sx);
