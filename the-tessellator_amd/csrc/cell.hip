@@ -851,8 +851,11 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   double r2 = cell_r2_slot(C, lane, &r2_slot);
 
   uint32_t streamed = 0, classified = 0, ncuts = 0, ninc = 0, status = 0;
-  uint32_t cls_base = 0;  // classifications already in flops
-  uint64_t flops = 24 + 40;
+  // the operation count in 32-bit scalar parts, combined at the end:
+  // 13 + 6V per classification (V the vertex count it was tested against),
+  // 12 per new vertex + 5V per cut
+  uint32_t cls_base = 0;  // classifications already in f_nv
+  uint32_t f_nv = 0, f_cut = 0;
   int maxv = 8;
   int e_lastcut = -1;
   const bool skip_cells = A.target_group < 0;
@@ -1085,12 +1088,11 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         }
         ncuts++;
         ninc += newv == 0;  // no Inside-Outside edge: only Incident vertices border the cut
-        // 13 + 6V per classification, V the vertex count it was tested against
-        flops += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
+        f_nv += (classified - cls_base) * (uint32_t)nv;
         cls_base = classified;
         nv = popc_multi(C.VA);
         if (nv > maxv) maxv = nv;
-        flops += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
+        f_cut += 12u * (uint32_t)newv + 5u * (uint32_t)nv;
         // the largest |v|^2 changes only if its vertex was cut off or a new
         // vertex exceeds it (new vertices lie on old edges, but rounding
         // may put one an ulp beyond)
@@ -1126,7 +1128,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           pv &= ballot(reach);
         }
       }
-      flops += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
+      f_nv += (classified - cls_base) * (uint32_t)nv;
       cls_base = classified;
       if (!skip_cells) {
         // the valid positions consumed that lie before the stop entry (a
@@ -1167,7 +1169,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       done = true;
     }
   }
-  flops += 8ull * streamed;
+  const uint64_t flops = 64ull + 8ull * streamed + 13ull * classified + 6ull * f_nv + f_cut;
 
   // ---- outputs
   const uint32_t oi = ext ? (uint32_t)gpos : self;
@@ -1192,7 +1194,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     else if (lane == 0)
       A.fpos[oi] = fbase;
   }
-  uint64_t oflops = 0;
+  uint32_t oflops = 0;  // this lane's faces
   if ((status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_ARENA)) == 0 && !table_rerun) {
     double area[FPL], contrib[FPL];
     int32_t tagv[FPL];
@@ -1248,7 +1250,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         seqv[r] = S.f_seq[f];
         hanc[r] = ha;
         ringk[r] = k;
-        oflops += 3ull * (k - 1) + 12ull * (k - 2) + 13;
+        oflops += 3u * (k - 1) + 12u * (k - 2) + 13u;
       }
     }
     // rank faces by creation number (volume order) and by tag (output order)
@@ -1379,7 +1381,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   } else {
     nfaces = 0;
   }
-  oflops = wave_sum_u64(oflops);
+  const uint64_t oflops_all = (uint32_t)rl_i(wave_incl_scan((int)oflops, lane), 63);
   if (lane == 0) {
     if (GEO != 1) {
       if (status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE | TESS_CELL_ARENA)) {
@@ -1390,7 +1392,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
     }
     if (A.summary) {
       if (!(status & (TESS_CELL_CAPACITY | TESS_CELL_DEGENERATE)) && !table_rerun) {
-        atomicAdd(&A.summary->flops, (unsigned long long)(flops + oflops + 1));
+        atomicAdd(&A.summary->flops, (unsigned long long)(flops + oflops_all + 1));
         atomicAdd(&A.summary->candidates, (unsigned long long)streamed);
         if (ninc) atomicAdd(&A.summary->incident_cuts, (unsigned long long)ninc);
       }
@@ -1404,7 +1406,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       st.classified = classified;
       st.cuts = ncuts;
       st.entries = (uint32_t)e_stop_final;
-      st.flops = flops + oflops + 1;
+      st.flops = flops + oflops_all + 1;
       st.r2 = r2;
       st.max_vertices = (uint32_t)maxv;
       st.status = status;
