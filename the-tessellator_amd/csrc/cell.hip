@@ -258,6 +258,7 @@ __constant__ int32_t kWallTag[6] = {-3, -2, -4, -1, -6, -5};
 template <int VPL, int PPL, int FPL>
 struct Lds {
   static constexpr int VC = 64 * VPL, PC = 64 * PPL, HC = 2 * PC, FC = 64 * FPL;
+  static_assert((VC & (VC - 1)) == 0, "vertex slot numbers are masked with VC - 1");
   // a pair slot number: one byte while there are at most 256 pairs
   using PairSlot = typename std::conditional<(PC > 256), uint16_t, uint8_t>::type;
   // a half-edge number: one byte while there are at most 256 half-edges
@@ -1420,7 +1421,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
 }
 
 // Wavefronts per SIMD the register allocation aims at: six for the
-// capacity-64 kernels (6,784 B of LDS each, 24 per CU), five for the larger
+// capacity-64 kernels (5,760 B of LDS each, up to 28 per CU), five for the larger
 // ones (their LDS caps them lower).
 template <int VPL> constexpr int kOcc = VPL == 1 ? 6 : 5;
 template <int VPL> constexpr int kOccItems = VPL == 1 ? 6 : 5;
