@@ -781,7 +781,10 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
 // One cell: the site at grid position gpos of the set A.  Returns the
 // cell's status bits (wave-uniform); the caller queues capacity reruns.
 // GEO: 0 the cell outputs, 1 the staged geometry only, 2 both.
-template <int VPL, int PPL, int FPL, int GEO = 0>
+// WIDE: the operation count accumulates in one 64-bit register pair instead
+// of 32-bit scalar parts (the same count; which form compiles to the faster
+// kernel differs between the plain and the batched launch).
+template <int VPL, int PPL, int FPL, int GEO = 0, bool WIDE = false>
 __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos, Lds<VPL, PPL, FPL>& S,
                                               const int lane, const int slot = 0) {
   using L = Lds<VPL, PPL, FPL>;
@@ -857,6 +860,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
   // 12 per new vertex + 5V per cut
   uint32_t cls_base = 0;  // classifications already in f_nv
   uint32_t f_nv = 0, f_cut = 0;
+  uint64_t f_wide = 0;  // WIDE: 13 + 6V per classification + 12 new + 5V per cut
   int maxv = 8;
   int e_lastcut = -1;
   const bool skip_cells = A.target_group < 0;
@@ -1089,11 +1093,17 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         }
         ncuts++;
         ninc += newv == 0;  // no Inside-Outside edge: only Incident vertices border the cut
-        f_nv += (classified - cls_base) * (uint32_t)nv;
+        if (WIDE)
+          f_wide += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
+        else
+          f_nv += (classified - cls_base) * (uint32_t)nv;
         cls_base = classified;
         nv = popc_multi(C.VA);
         if (nv > maxv) maxv = nv;
-        f_cut += 12u * (uint32_t)newv + 5u * (uint32_t)nv;
+        if (WIDE)
+          f_wide += 12ull * (uint64_t)newv + 5ull * (uint64_t)nv;
+        else
+          f_cut += 12u * (uint32_t)newv + 5u * (uint32_t)nv;
         // the largest |v|^2 changes only if its vertex was cut off or a new
         // vertex exceeds it (new vertices lie on old edges, but rounding
         // may put one an ulp beyond)
@@ -1129,7 +1139,10 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
           pv &= ballot(reach);
         }
       }
-      f_nv += (classified - cls_base) * (uint32_t)nv;
+      if (WIDE)
+        f_wide += (uint64_t)(classified - cls_base) * (13ull + 6ull * (uint64_t)nv);
+      else
+        f_nv += (classified - cls_base) * (uint32_t)nv;
       cls_base = classified;
       if (!skip_cells) {
         // the valid positions consumed that lie before the stop entry (a
@@ -1170,7 +1183,8 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       done = true;
     }
   }
-  const uint64_t flops = 64ull + 8ull * streamed + 13ull * classified + 6ull * f_nv + f_cut;
+  const uint64_t flops = WIDE ? 64ull + 8ull * streamed + f_wide
+                              : 64ull + 8ull * streamed + 13ull * classified + 6ull * f_nv + f_cut;
 
   // ---- outputs
   const uint32_t oi = ext ? (uint32_t)gpos : self;
@@ -1434,7 +1448,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOcc<VPL>)))
   const int w = A.xcd_chunk ? (int)xcd_map(blockIdx.x, (uint32_t)A.xcd_chunk) : (int)blockIdx.x;
   if (w >= A.nwork) return;
   const int gpos = A.work ? (int)A.work[w] : w;
-  const uint32_t status = cell_body<VPL, PPL, FPL, GEO>(A, gpos, S, lane, w);
+  const uint32_t status = cell_body<VPL, PPL, FPL, GEO, true>(A, gpos, S, lane, w);
   if (lane == 0 && (status & TESS_CELL_CAPACITY) && A.redo) {
     const uint32_t slot = atomicAdd(A.nredo, 1u);
     A.redo[slot] = (uint32_t)gpos;
