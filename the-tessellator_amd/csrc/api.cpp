@@ -374,21 +374,39 @@ tess_status compute_cells(tess_diagram* d, const Planner* plan) {
   for (int slot = 2; slot >= 0; --slot)
     if (sch.count[slot] && (st = run_cells(d, t, kVpl[slot], sch.list[slot], sch.count[slot], true)) != TESS_OK)
       return st;
-  // cells that outgrew their capacity: rerun with the largest variant
+  // cells that outgrew their capacity: rerun with the largest variant; after
+  // a 64-vertex whole-set pass, first at 128 vertices (nearly all of them
+  // fit, and that variant keeps more wavefronts resident) and the rest at 256
   uint32_t nredo = 0;
   if ((st = c->hip(hipMemcpyAsync(&nredo, d->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
     return st;
   if (nredo) {
     if (d->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
-    if (GeoStage* g = d->geo_stage) {  // staging of the rerun items
-      g->nredo = nredo;
-      if ((st = geo_staging_alloc(d, g->rerun, nredo, 4)) != TESS_OK) return st;
-    }
     if ((st = c->hip(hipMemcpyAsync(d->redo_list.p, d->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
-                     "redo")) != TESS_OK ||
-        (st = run_cells(d, t, 4, d->redo_list.p, (int)nredo, true)) != TESS_OK)
+                     "redo")) != TESS_OK)
       return st;
+    if (!plan && !d->geo_stage && pick_vpl(d) == 1) {
+      uint32_t nredo2 = 0;
+      if ((st = c->hip(launch_fill_u32(d->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+          (st = run_cells(d, t, 2, d->redo_list.p, (int)nredo, true)) != TESS_OK ||
+          (st = c->hip(hipMemcpyAsync(&nredo2, d->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
+          (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
+        return st;
+      if (nredo2) {
+        if (d->redo_list2.alloc(nredo2) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+        if ((st = c->hip(hipMemcpyAsync(d->redo_list2.p, d->redo.p + 1, 4ull * nredo2, hipMemcpyDeviceToDevice, s),
+                         "redo")) != TESS_OK ||
+            (st = run_cells(d, t, 4, d->redo_list2.p, (int)nredo2, true)) != TESS_OK)
+          return st;
+      }
+    } else {
+      if (GeoStage* g = d->geo_stage) {  // staging of the rerun items
+        g->nredo = nredo;
+        if ((st = geo_staging_alloc(d, g->rerun, nredo, 4)) != TESS_OK) return st;
+      }
+      if ((st = run_cells(d, t, 4, d->redo_list.p, (int)nredo, true)) != TESS_OK) return st;
+    }
   }
   // cells past the shell table or the largest LDS capacity
   std::vector<uint32_t> tab, spl;

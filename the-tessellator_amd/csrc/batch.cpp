@@ -74,7 +74,7 @@ struct tess_batch {
   DevBuf<GridDev> grid;
   DevBuf<uint32_t> bins, cnt, delim, cursor, sorted, scratch, sums, big, misc, fcount, status, ctrl;
   DevBuf<int32_t> snbr, nbr;
-  DevBuf<uint64_t> foff, fsums, items, redo, base_dev, dend_dev;
+  DevBuf<uint64_t> foff, fsums, items, redo, redo2, base_dev, dend_dev;
   // face arena of all sets, shell-table exhaustions, cells for the spill kernel
   DevBuf<uint64_t> fpos, tredo, spill;
   DevBuf<unsigned long long> arena;
@@ -109,9 +109,10 @@ namespace {
 
 // ctrl words: [0] rerun count, [1] rerun counter, [2] binning check, [3..8]
 // slot bounds of the learned order (and its radius word), [10] shell-table
-// exhaustions, [11] cells for the spill kernel
+// exhaustions, [11] cells for the spill kernel, [12] count and [13] counter
+// of the second rerun (cells past 128 vertices)
 constexpr int kCtrlRedo = 0, kCtrlQueue = 1, kCtrlBad = 2, kCtrlBounds = 3, kCtrlTab = 10, kCtrlSpill = 11,
-              kCtrlWords = 12;
+              kCtrlRedo2 = 12, kCtrlQueue2 = 13, kCtrlWords = 14;
 
 tess_status make_views(tess_batch* b) {
   tess_ctx* c = b->ctx;
@@ -315,7 +316,7 @@ tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32
   tess_ctx* c = b->ctx;
   hipStream_t s = c->stream;
   tess_status st;
-  if (launch_fill_u32(b->ctrl.p, 2, 0u, s) != hipSuccess || launch_fill_u32(b->ctrl.p + kCtrlTab, 2, 0u, s) != hipSuccess ||
+  if (launch_fill_u32(b->ctrl.p, 2, 0u, s) != hipSuccess || launch_fill_u32(b->ctrl.p + kCtrlTab, 4, 0u, s) != hipSuccess ||
       launch_fill_u32((uint32_t*)b->arena.p, 2 * kArenaWords, 0u, s) != hipSuccess ||
       launch_fill_u32((uint32_t*)b->summary.p, b->nsets * (sizeof(Summary) / 4), 0u, s) != hipSuccess)
     return c->fail(TESS_E_DEVICE, "fill");
@@ -325,8 +326,8 @@ tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32
   a.items = b->items.p;
   a.tredo = b->tredo.p;
   a.ntredo = b->ctrl.p + kCtrlTab;
-  // capacity overflows below 256 vertices go to the rerun list, at 256 to
-  // the spill kernel's
+  // capacity overflows of the first passes below 256 vertices go to the
+  // rerun list, at 256 to the spill kernel's
   auto overflow_to = [&](BatchArgs& x, int vpl) {
     x.redo = vpl < 4 ? b->redo.p : b->spill.p;
     x.nredo = b->ctrl.p + (vpl < 4 ? kCtrlRedo : kCtrlSpill);
@@ -348,21 +349,33 @@ tess_status enqueue_cells(tess_batch* b, const uint32_t* list_base, const uint32
     a.xcd_chunk = b->cfg.xcd_chunk;
     if ((st = c->hip(launch_cells_items(vpl, a, (uint32_t)b->N, s), "cell pass")) != TESS_OK) return st;
   }
-  // rerun at the largest capacity: persistent over the device-counted list
-  BatchArgs r;
-  std::memset(&r, 0, sizeof(r));
-  r.sets = b->args.p;
-  r.items = b->redo.p;
-  r.count = b->ctrl.p + kCtrlRedo;
-  r.counter = b->ctrl.p + kCtrlQueue;
-  r.chunk = std::max(1, b->cfg.redo_chunk);
-  overflow_to(r, 4);
-  r.tredo = b->tredo.p;
-  r.ntredo = b->ctrl.p + kCtrlTab;
+  // reruns, persistent over device-counted lists: at 128 vertices (the
+  // cells that outgrew 64 nearly all fit, and this variant keeps more
+  // wavefronts resident), then at 256 for the cells past 128
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0)
     ncu = 256;
-  return c->hip(launch_cells_queue(r, ncu * cells_queue_blocks_per_cu(), s), "rerun pass");
+  for (int stage = 0; stage < 2; ++stage) {
+    const int vpl = stage == 0 ? 2 : 4;
+    BatchArgs r;
+    std::memset(&r, 0, sizeof(r));
+    r.sets = b->args.p;
+    r.items = stage == 0 ? b->redo.p : b->redo2.p;
+    r.count = b->ctrl.p + (stage == 0 ? kCtrlRedo : kCtrlRedo2);
+    r.counter = b->ctrl.p + (stage == 0 ? kCtrlQueue : kCtrlQueue2);
+    r.chunk = std::max(1, b->cfg.redo_chunk);
+    if (stage == 0) {  // past 128 vertices: to the second rerun
+      r.redo = b->redo2.p;
+      r.nredo = b->ctrl.p + kCtrlRedo2;
+    } else {
+      overflow_to(r, 4);
+    }
+    r.tredo = b->tredo.p;
+    r.ntredo = b->ctrl.p + kCtrlTab;
+    if ((st = c->hip(launch_cells_queue(vpl, r, ncu * cells_queue_blocks_per_cu(vpl), s), "rerun pass")) != TESS_OK)
+      return st;
+  }
+  return TESS_OK;
 }
 
 // Cells of one set that the batched passes could not finish (shell table,
@@ -601,7 +614,7 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
             b->volume.alloc(N) == hipSuccess && b->fcount.alloc(N) == hipSuccess && b->status.alloc(N) == hipSuccess &&
             b->snbr.alloc(arena_capacity(N)) == hipSuccess && b->sarea.alloc(arena_capacity(N)) == hipSuccess &&
             b->foff.alloc(N + 1) == hipSuccess && b->fsums.alloc((N + 1023) / 1024 + 1) == hipSuccess &&
-            b->items.alloc(N) == hipSuccess && b->redo.alloc(N) == hipSuccess &&
+            b->items.alloc(N) == hipSuccess && b->redo.alloc(N) == hipSuccess && b->redo2.alloc(N) == hipSuccess &&
             b->base_dev.alloc(nsets + 1) == hipSuccess && b->dend_dev.alloc(nsets) == hipSuccess &&
             b->summary.alloc(nsets) == hipSuccess && b->args.alloc(nsets) == hipSuccess &&
             b->ctrl.alloc(kCtrlWords) == hipSuccess && b->fpos.alloc(N) == hipSuccess &&

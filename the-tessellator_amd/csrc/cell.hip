@@ -1493,7 +1493,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kOccItems<VP
 // (*B.count): every wavefront takes `chunk` items at a time from a shared
 // counter until the list is exhausted, so the loop ends for every wavefront.
 template <int VPL, int PPL, int FPL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_cells_queue(BatchArgs B) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VPL == 2 ? 4 : 3))) void k_cells_queue(BatchArgs B) {
   __shared__ Lds<VPL, PPL, FPL> S;
   const int lane = threadIdx.x;
   const uint32_t n = *B.count;
@@ -1993,6 +1993,7 @@ template __global__ void k_cells<4, 6, 2>(CellArgs);
 template __global__ void k_cells_items<1, 2, 1>(BatchArgs, uint32_t);
 template __global__ void k_cells_items<2, 3, 2>(BatchArgs, uint32_t);
 template __global__ void k_cells_items<4, 6, 2>(BatchArgs, uint32_t);
+template __global__ void k_cells_queue<2, 3, 2>(BatchArgs);
 template __global__ void k_cells_queue<4, 6, 2>(BatchArgs);
 
 // f64 sqrt / reciprocal / division as compiled for the cell kernel.
@@ -2027,15 +2028,19 @@ hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_cells_queue(const BatchArgs& b, int grid, hipStream_t s) {
-  if (grid <= 0 || b.chunk <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_cells_queue<4, 6, 2>), dim3(grid), dim3(64), 0, s, b);
+hipError_t launch_cells_queue(int vpl, const BatchArgs& b, int grid, hipStream_t s) {
+  if (grid <= 0 || b.chunk <= 0 || (vpl != 2 && vpl != 4)) return hipErrorInvalidValue;
+  if (vpl == 2)
+    hipLaunchKernelGGL((k_cells_queue<2, 3, 2>), dim3(grid), dim3(64), 0, s, b);
+  else
+    hipLaunchKernelGGL((k_cells_queue<4, 6, 2>), dim3(grid), dim3(64), 0, s, b);
   return hipGetLastError();
 }
 
-int cells_queue_blocks_per_cu() {
+int cells_queue_blocks_per_cu(int vpl) {
   int nb = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cells_queue<4, 6, 2>, 64, 0);
+  hipError_t e = vpl == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cells_queue<2, 3, 2>, 64, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cells_queue<4, 6, 2>, 64, 0);
   return e == hipSuccess && nb > 0 ? nb : 1;
 }
 

@@ -56,10 +56,10 @@ hipError_t launch_geo_compact(const uint32_t* slot_cell, int nslots, int vcap, i
                               uint16_t* out_ring, uint64_t* out_roff, hipStream_t s);
 // many sets: one wavefront per item of a list of host-known length n ...
 hipError_t launch_cells_items(int vpl, const BatchArgs& b, uint32_t n, hipStream_t s);
-// ... or persistent over a device-counted list at the largest capacity
-// (grid = resident blocks: cells_queue_blocks_per_cu() x CUs)
-hipError_t launch_cells_queue(const BatchArgs& b, int grid, hipStream_t s);
-int cells_queue_blocks_per_cu();
+// ... or persistent over a device-counted list at capacity 128 (vpl 2) or
+// 256 (vpl 4) (grid = resident blocks: cells_queue_blocks_per_cu(vpl) x CUs)
+hipError_t launch_cells_queue(int vpl, const BatchArgs& b, int grid, hipStream_t s);
+int cells_queue_blocks_per_cu(int vpl);
 // cells too large for the LDS kernels, one block per workspace (p.wsd ...
 // hold `blocks` workspaces), blocks taking items in turn
 hipError_t launch_cells_spill(const SpillArgs& p, int blocks, hipStream_t s);

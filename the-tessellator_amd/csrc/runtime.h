@@ -190,7 +190,7 @@ struct tess_diagram {
   DevBuf<uint64_t> foff, fsums;
   DevBuf<tess_cell_stats> stats;
   DevBuf<Summary> summary;
-  DevBuf<uint32_t> work1, redo_list;
+  DevBuf<uint32_t> work1, redo_list, redo_list2;
   // cells that exhausted the shell table / outgrew the largest LDS variant:
   // count at [0], grid positions from [1]
   DevBuf<uint32_t> tredo, spill;
