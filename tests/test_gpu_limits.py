@@ -122,6 +122,33 @@ def test_sphere_ring_spills(ctx, capacity):
     assert abs(got["volume"].sum() - 1.0) <= 1e-10
 
 
+@pytest.mark.parametrize("batched", [False, True])
+def test_sphere_ring_of_100_reruns_twice(ctx, batched):
+    """100 faces, about 196 vertices: past the first pass (64 vertices) and
+    the first rerun (128), finished by the second rerun at 256 vertices."""
+    pts = sphere_ring(m=100, seed=13, r=0.3, bg=3000)
+    ref = orc.compute(pts, LO, HI)
+    if batched:
+        for eager in (0, 1):
+            b = tess.Batch(ctx, [(pts, LO, HI), (tess.uniform(2000, 14), LO, HI)])
+            b.set_config(eager=eager)
+            b.compute()
+            same_cells(b.cells(0), ref)
+            assert b.timings()["capacity_reruns"] >= 1 and b.timings()["spill_cells"] == 0
+            b.close()
+        return
+    d = tess.Diagram(ctx, pts, LO, HI, collect_stats=True)
+    d.compute()
+    got, st = d.cells(), d.stats()
+    same_cells(got, ref)
+    same_counters(st, ref["stats"])
+    assert got["face_count"][0] == 100
+    assert 128 < st["max_vertices"][0] <= 256
+    assert st["status"][0] & tess.CELL_RERUN
+    assert not st["status"][0] & tess.CELL_SPILL
+    assert d.timings()["spill_cells"] == 0
+
+
 def test_sphere_ring_of_600_grows_the_workspace(ctx):
     """600 faces, about 1200 vertices: past the first spill workspace (512
     faces), so the kernel reruns the cell with a larger one."""
