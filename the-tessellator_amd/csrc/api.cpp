@@ -142,9 +142,8 @@ tess_status alloc_all(tess_diagram* d) {
   return TESS_OK;
 }
 
-tess_status bin_enqueue(tess_diagram* d) {
+tess_status bin_enqueue(tess_diagram* d, hipStream_t s) {
   tess_ctx* c = d->ctx;
-  hipStream_t s = c->stream;
   const int n = (int)d->n;
   const int nparts = (int)std::min<size_t>(256, (d->n + 255) / 256);
   tess_status st;
@@ -169,7 +168,7 @@ tess_status bin(tess_diagram* d) {
   tess_ctx* c = d->ctx;
   hipStream_t s = c->stream;
   tess_status st;
-  if ((st = bin_enqueue(d)) != TESS_OK) return st;
+  if ((st = bin_enqueue(d, s)) != TESS_OK) return st;
   if ((st = c->hip(hipMemcpyAsync(&d->hgrid, d->grid.p, sizeof(GridDev), hipMemcpyDeviceToHost, s),
                    "grid copy")) != TESS_OK)
     return st;

@@ -100,6 +100,19 @@ struct tess_batch {
   std::vector<TableKey> table_of;  // per set, as uploaded
   Graph gbin, gcells;
   std::vector<int64_t> gcells_key;  // config + slot counts the cells graph was captured for
+  // side streams the sets' binning is spread over (default priority), and
+  // the events that fork them from and join them back into the context's
+  // stream
+  struct Side {
+    std::vector<hipStream_t> s;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    ~Side() {
+      for (hipStream_t x : s) (void)hipStreamDestroy(x);
+      for (hipEvent_t e : join) (void)hipEventDestroy(e);
+      if (fork) (void)hipEventDestroy(fork);
+    }
+  } side;
   uint64_t total_faces = 0;
   tess_timings tim;
   bool computed = false;
@@ -175,17 +188,52 @@ tess_status capture_end(tess_ctx* c, Graph& g) {
   return st;
 }
 
+// The binning of every set spread over side streams forked from the
+// context's stream and joined back into it (the sets' binning buffers are
+// disjoint slices, so their short kernel chains can run side by side; under
+// capture the fork and the joins become graph edges), then the check.
+constexpr size_t kBinStreams = 8;
+
+tess_status enqueue_binning(tess_batch* b) {
+  tess_ctx* c = b->ctx;
+  hipStream_t s = c->stream;
+  tess_status st;
+  const size_t k = std::min(kBinStreams, b->nsets);
+  if (b->side.s.size() < k) {
+    if (!b->side.fork && (st = c->hip(hipEventCreateWithFlags(&b->side.fork, hipEventDisableTiming), "event")) != TESS_OK)
+      return st;
+    while (b->side.s.size() < k) {
+      hipStream_t x = nullptr;
+      hipEvent_t e = nullptr;
+      if ((st = c->hip(hipStreamCreateWithFlags(&x, hipStreamNonBlocking), "stream")) != TESS_OK) return st;
+      b->side.s.push_back(x);
+      if ((st = c->hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event")) != TESS_OK) return st;
+      b->side.join.push_back(e);
+    }
+  }
+  if ((st = c->hip(launch_fill_u32(b->ctrl.p + kCtrlBad, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(hipEventRecord(b->side.fork, s), "event")) != TESS_OK)
+    return st;
+  tess_status err = TESS_OK;
+  for (size_t j = 0; j < k && err == TESS_OK; ++j) err = c->hip(hipStreamWaitEvent(b->side.s[j], b->side.fork, 0), "fork");
+  for (size_t set = 0; set < b->nsets && err == TESS_OK; ++set) err = bin_enqueue(b->views[set].get(), b->side.s[set % k]);
+  // every side stream joins back, also after an error (a capture must end joined)
+  for (size_t j = 0; j < k; ++j) {
+    tess_status e1 = c->hip(hipEventRecord(b->side.join[j], b->side.s[j]), "join");
+    tess_status e2 = c->hip(hipStreamWaitEvent(s, b->side.join[j], 0), "join");
+    if (err == TESS_OK) err = e1 != TESS_OK ? e1 : e2;
+  }
+  if (err != TESS_OK) return err;
+  return c->hip(launch_batch_check(b->delim.p, b->dend_dev.p, b->base_dev.p, (int)b->nsets, b->ctrl.p + kCtrlBad, s),
+                "binning check");
+}
+
 // Binning of every set, then the check; captured once.
 tess_status build_bin_graph(tess_batch* b) {
   tess_ctx* c = b->ctx;
   tess_status st = capture_begin(c);
   if (st != TESS_OK) return st;
-  tess_status err = launch_fill_u32(b->ctrl.p + kCtrlBad, 1, 0u, c->stream) == hipSuccess ? TESS_OK : TESS_E_DEVICE;
-  for (size_t s = 0; s < b->nsets && err == TESS_OK; ++s) err = bin_enqueue(b->views[s].get());
-  if (err == TESS_OK)
-    err = c->hip(launch_batch_check(b->delim.p, b->dend_dev.p, b->base_dev.p, (int)b->nsets, b->ctrl.p + kCtrlBad,
-                                    c->stream),
-                 "binning check");
+  tess_status err = enqueue_binning(b);
   st = capture_end(c, b->gbin);
   return err != TESS_OK ? err : st;
 }
@@ -413,14 +461,7 @@ tess_status batch_compute(tess_batch* b, const tess_model* m) {
   Mail* mail = b->mail;
   if (eager) {
     // the binning passes of every set, launched directly
-    if ((st = c->hip(hipEventRecord(ev[0], s), "event")) != TESS_OK ||
-        (st = c->hip(launch_fill_u32(b->ctrl.p + kCtrlBad, 1, 0u, s), "fill")) != TESS_OK)
-      return st;
-    for (size_t k = 0; k < b->nsets; ++k)
-      if ((st = bin_enqueue(b->views[k].get())) != TESS_OK) return st;
-    if ((st = c->hip(launch_batch_check(b->delim.p, b->dend_dev.p, b->base_dev.p, (int)b->nsets, b->ctrl.p + kCtrlBad,
-                                        s),
-                     "binning check")) != TESS_OK)
+    if ((st = c->hip(hipEventRecord(ev[0], s), "event")) != TESS_OK || (st = enqueue_binning(b)) != TESS_OK)
       return st;
   } else {
     if (!b->gbin.exec && (st = build_bin_graph(b)) != TESS_OK) return st;

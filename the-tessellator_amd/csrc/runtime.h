@@ -245,7 +245,7 @@ namespace tess {
 
 tess_status alloc_all(tess_diagram* d);
 // bounds -> counting sort -> gather, enqueued only (graph-capturable)
-tess_status bin_enqueue(tess_diagram* d);
+tess_status bin_enqueue(tess_diagram* d, hipStream_t s);
 // bin_enqueue + grid download + the check that every point was binned
 tess_status bin(tess_diagram* d);
 int pick_vpl(const tess_diagram* d);
