@@ -54,9 +54,9 @@ __device__ __forceinline__ int mbcnt(uint64_t m, int lane) {
   (void)lane;
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-// Keeps v a per-lane value the compiler takes as it is: a ballot of a test
-// on it stays one vector compare instead of a scalar re-derivation from the
-// masks v came from.
+// An empty asm that hands v back as an opaque per-lane value: the compiler
+// keeps the slot ranks below as vector values and does not fold their
+// compare into the masks they came from.
 __device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
 // Bit `lane` of a wave-uniform mask as this lane's predicate: the mask is
 // used as the lane mask it is, with no per-lane shift.
