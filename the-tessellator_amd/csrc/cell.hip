@@ -235,9 +235,9 @@ __device__ __forceinline__ uint64_t arena_reserve(unsigned long long* ctr, uint6
 
 // A chunk with at least this many staged candidates is pre-tested lane-parallel.
 constexpr int kPreMin = 16;
-// A chunk looks at up to 64 * kWin stream positions (kWin per lane) and
-// stages the first 64 kept candidates among them.
-constexpr int kWin = 2;
+// A chunk looks at up to 64 * WIN stream positions (WIN per lane, WIN at
+// most kWinMax) and stages the first 64 kept candidates among them.
+constexpr int kWinMax = 4;
 
 __device__ __forceinline__ bool lex_less(double ax, double ay, double az, double bx, double by, double bz) {
   if (ax != bx) return ax < bx;
@@ -273,7 +273,7 @@ struct Lds {
       int32_t tag[64];
       int32_t ent[64];
     } cand;
-    uint8_t head[64 * kWin];  // the entry whose candidates start at window position p (0xff: none)
+    uint8_t head[64 * kWinMax];  // the entry whose candidates start at window position p (0xff: none)
     double fcontrib[FC > 64 ? FC : 64];
   } u;
   int32_t f_tag[FC];
@@ -783,12 +783,14 @@ __device__ double cell_r2_slot(const CellState<VPL, PPL, FPL>& C, int lane, int*
 // GEO: 0 the cell outputs, 1 the staged geometry only, 2 both.
 // WIDE: the operation count accumulates in one 64-bit register pair instead
 // of 32-bit scalar parts (the same count; which form compiles to the faster
-// kernel differs between the plain and the batched launch).
-template <int VPL, int PPL, int FPL, int GEO = 0, bool WIDE = false>
+// kernel differs between the plain and the batched launch).  WIN: stream
+// slices per chunk (2 for the plain launch, 4 for the batched one).
+template <int VPL, int PPL, int FPL, int GEO = 0, bool WIDE = false, int WIN = 2>
 __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos, Lds<VPL, PPL, FPL>& S,
                                               const int lane, const int slot = 0) {
   using L = Lds<VPL, PPL, FPL>;
   constexpr int FC = L::FC;
+  static_assert(WIN >= 1 && WIN <= kWinMax, "the window's heads fit the staging space");
   CellState<VPL, PPL, FPL> C;
   // a site of the set excludes itself by index (interface.rs:299-305); an
   // outside site (interface.rs:211-232) excludes nothing
@@ -926,21 +928,21 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       const int pre = incl - c;
       S.cs.st.bpref[lane] = (uint32_t)pre;
       S.cs.st.bstart[lane] = bst + (uint32_t)(lane == ce ? co : 0);
-      // The window: stream positions [0, min(total, 64 kWin)), lane + 64 j
+      // The window: stream positions [0, min(total, 64 WIN)), lane + 64 j
       // in slice j.  Entry of position k: the last non-empty entry starting
       // at or before k (heads by position, then a running maximum carried
       // across the slices).
-      const int nsl = total < 64 * kWin ? (total + 63) >> 6 : kWin;
+      const int nsl = total < 64 * WIN ? (total + 63) >> 6 : WIN;
 #pragma unroll
-      for (int j = 0; j < kWin; ++j) S.u.head[lane + 64 * j] = 0xff;
+      for (int j = 0; j < WIN; ++j) S.u.head[lane + 64 * j] = 0xff;
       wave_sync();
-      if (c > 0 && pre < 64 * kWin) S.u.head[pre] = (uint8_t)lane;
+      if (c > 0 && pre < 64 * WIN) S.u.head[pre] = (uint8_t)lane;
       wave_sync();
-      int ekw[kWin];
+      int ekw[WIN];
       {
         int carry = -1;
 #pragma unroll
-        for (int j = 0; j < kWin; ++j) {
+        for (int j = 0; j < WIN; ++j) {
           ekw[j] = 0;
           if (j >= nsl) continue;
           const int hk = S.u.head[lane + 64 * j];
@@ -954,11 +956,11 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
       // Stage the kept candidates in stream order, up to 64: the chunk ends
       // after the 64th (positions past it are streamed again by the next
       // chunk).  The sequential pass reads them back as wave-uniform values.
-      int consumed = total < 64 * kWin ? total : 64 * kWin;
+      int consumed = total < 64 * WIN ? total : 64 * WIN;
       int nkeep = 0;
-      uint64_t vmw[kWin];  // valid positions of each slice (group-filter streamed count)
+      uint64_t vmw[WIN];  // valid positions of each slice (group-filter streamed count)
 #pragma unroll
-      for (int j = 0; j < kWin; ++j) {
+      for (int j = 0; j < WIN; ++j) {
         vmw[j] = 0;
         if (64 * j >= consumed) continue;
         const int k = lane + 64 * j;
@@ -1151,7 +1153,7 @@ __device__ __forceinline__ uint32_t cell_body(const CellArgs& A, const int gpos,
         const int pe = es >= 64 ? consumed : rl_i(pre, es);
         const int lim_k = pe < consumed ? pe : consumed;
 #pragma unroll
-        for (int j = 0; j < kWin; ++j) {
+        for (int j = 0; j < WIN; ++j) {
           const int lk = lim_k - 64 * j;
           if (lk <= 0) break;
           streamed += (uint32_t)popc(lk >= 64 ? vmw[j] : (vmw[j] & ((1ull << lk) - 1ull)));
@@ -1467,7 +1469,7 @@ __device__ __forceinline__ void batch_item(const BatchArgs& B, uint64_t it, Lds<
   typedef const __attribute__((address_space(4))) CellArgs ConstArgs;
   ConstArgs* cs = (ConstArgs*)(B.sets) + (uint32_t)(it >> 32);
   const CellArgs& A = *(const CellArgs*)cs;
-  const uint32_t status = cell_body(A, (int)(uint32_t)it, S, lane);
+  const uint32_t status = cell_body<VPL, PPL, FPL, 0, false, 4>(A, (int)(uint32_t)it, S, lane);
   if (lane == 0 && (status & TESS_CELL_CAPACITY) && B.redo) {
     const uint32_t slot = atomicAdd(B.nredo, 1u);
     B.redo[slot] = it;
