@@ -46,6 +46,29 @@ def test_binning_matches_oracle(ctx, n, seed, density):
     assert np.array_equal(delim, dl)
 
 
+@pytest.mark.parametrize("order", [tess.ORDER_INPUT, tess.ORDER_MORTON])
+def test_binning_dense_cells_matches_oracle(ctx, order):
+    """Grid cells far past the small-cell insertion sort: a ball of 6,000
+    points inside one grid cell (larger than the LDS-staged ranking holds, so
+    the global ranking runs; in Morton order all of them share one code and
+    tie by index) and a Gaussian cluster whose cells hold tens to hundreds of
+    points (the LDS-staged ranking), in a uniform background, shuffled."""
+    rng = np.random.default_rng(11)
+    ball = np.array([0.31, 0.52, 0.77]) + rng.uniform(-1e-4, 1e-4, (6000, 3))
+    clus = np.clip(np.array([0.6, 0.3, 0.4]) + rng.normal(0.0, 0.03, (4000, 3)), 0.0, 1.0)
+    pts = rng.permutation(np.concatenate([rng.uniform(0.0, 1.0, (20000, 3)), ball, clus]))
+    d = tess.Diagram(ctx, pts, UNIT_LO, UNIT_HI, density=5.0, point_order=order)
+    d.compute()
+    _, bins, srt, dl = orc.grid(pts, 5.0, order=order)
+    counts = np.diff(dl)
+    assert counts.max() > 4096 and np.count_nonzero((counts > 32) & (counts <= 4096)) >= 10
+    cell, sorted_, delim = d.copy_grid()
+    assert np.array_equal(cell, bins)
+    assert np.array_equal(delim, dl)
+    assert np.array_equal(sorted_, srt)
+    d.close()
+
+
 def test_single_point_closed_form(ctx):
     d = tess.Diagram(ctx, np.array([[0.3, 0.6, 0.2]]), UNIT_LO, UNIT_HI)
     d.compute()

@@ -7,7 +7,8 @@
 //                atomics are aggregated per run of equal ids in a wavefront
 //   k_scan_*     exclusive prefix scan of the histogram = delimiters
 //                (celery.rs:372-414)
-//   k_scatter    points to their cell's range (atomic cursor)
+//   k_scatter    points to their cell's range (atomic cursor), or, with the
+//                ranks the histogram's atomics hand out, without atomics
 //   k_sort_cells ascending point index inside each cell (stable order, the
 //                canonical form of celery.rs:357-369), or ascending
 //                (Morton code, index) for the internal order
@@ -117,12 +118,16 @@ __global__ __launch_bounds__(64) void k_bounds_final(const double* __restrict__ 
 
 // Cell id of every point; histogram with one atomic per run of equal ids
 // among consecutive lanes (spatially ordered inputs collapse to few atomics).
+// rank (nullable): each point's slot inside its cell's range, from the value
+// the run's atomic returns (a permutation of 0 .. count - 1 per cell, in no
+// set order: the in-cell sort fixes the order afterwards).
 __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z, int n,
                                                    const GridDev* __restrict__ gp,
                                                    uint32_t* __restrict__ bins,
-                                                   uint32_t* __restrict__ cnt) {
+                                                   uint32_t* __restrict__ cnt,
+                                                   uint32_t* __restrict__ rank) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const GridDev g = *gp;
   const int lane = __lane_id();
@@ -140,11 +145,20 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ x,
   // branch below would only see the head lanes)
   const uint64_t heads = __ballot(head);
   const uint64_t act = __ballot(i < n);
+  uint32_t first = 0;  // the run's first slot (head lanes)
   if (head) {
     const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
     const int last = 63 - __clzll((long long)act);  // last active lane
     const int nexth = after ? lane + 1 + (__ffsll((unsigned long long)after) - 1) : last + 1;
-    atomicAdd(&cnt[b], (uint32_t)(nexth - lane));
+    first = atomicAdd(&cnt[b], (uint32_t)(nexth - lane));
+  }
+  if (rank) {
+    // the run's head: the highest head lane at or below this one (an active
+    // lane always has one: the first active lane is a head)
+    const uint64_t upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+    const int h = upto ? 63 - __clzll((long long)upto) : lane;
+    const uint32_t f = (uint32_t)__shfl((int)first, h, 64);
+    if (i < n) rank[i] = f + (uint32_t)(lane - h);
   }
 }
 
@@ -230,6 +244,14 @@ __global__ void k_scatter(const uint32_t* __restrict__ bins, int n, uint32_t* __
   sorted[pos] = (uint32_t)i;
 }
 
+// The same with the slots k_bin_count handed out: no atomics.
+__global__ void k_scatter_ranked(const uint32_t* __restrict__ bins, const uint32_t* __restrict__ rank, int n,
+                                 const uint32_t* __restrict__ delim, uint32_t* __restrict__ sorted) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sorted[delim[bins[i]] + rank[i]] = (uint32_t)i;
+}
+
 // Ascending point index inside each cell: insertion sort for small cells,
 // one wavefront per cell (rank by comparison) for large ones.
 // Order of two points of a cell: (key, index) when keys are given (the
@@ -265,18 +287,48 @@ __global__ void k_sort_cells(const uint32_t* __restrict__ delim, int nbins,
   }
 }
 
-// Large cells: rank of each element = number of smaller ids in the cell
-// (ids are distinct), written into a scratch copy then back.
+// Large cells: rank of each element = number of elements before it in the
+// cell's order (ids are distinct).  A cell of at most kSortLds points is
+// staged in LDS (ids and keys) and ranked against the staged copy, each
+// element written straight to its rank; a larger one is ranked against
+// global memory into a scratch copy, then copied back.
+constexpr uint32_t kSortLds = 4096;
 __global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ delim,
                                                   const uint32_t* __restrict__ big,
                                                   const uint32_t* __restrict__ nbig,
                                                   uint32_t* __restrict__ sorted,
                                                   uint32_t* __restrict__ scratch,
                                                   const uint32_t* __restrict__ key) {
+  __shared__ uint32_t sid[kSortLds];
+  __shared__ uint32_t skey[kSortLds];
   const uint32_t nb = *nbig;
   for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
     const uint32_t b = big[q];
-    const uint32_t s = delim[b], e = delim[b + 1];
+    const uint32_t s = delim[b], e = delim[b + 1], m = e - s;
+    if (m <= kSortLds) {
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t v = sorted[s + i];
+        sid[i] = v;
+        if (key) skey[i] = key[v];
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t v = sid[i];
+        uint32_t r = 0;
+        if (key) {
+          const uint32_t kv = skey[i];
+          for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t kj = skey[j];
+            r += (kj < kv || (kj == kv && sid[j] < v)) ? 1u : 0u;
+          }
+        } else {
+          for (uint32_t j = 0; j < m; ++j) r += sid[j] < v ? 1u : 0u;
+        }
+        sorted[s + r] = v;
+      }
+      __syncthreads();
+      continue;
+    }
     for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) {
       const uint32_t v = sorted[i];
       uint32_t r = 0;
@@ -415,14 +467,16 @@ hipError_t launch_counting_sort(const double* x, const double* y, const double* 
   hipError_t e;
   if ((e = launch_fill_u32(b.cnt, (size_t)nbins, 0u, s)) != hipSuccess) return e;
   if ((e = launch_fill_u32(b.misc, 4, 0u, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt);
+  // the ranks go to b.scratch, which the in-cell sort reuses once the
+  // scatter has read them
+  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt, b.scratch);
   const int nsb = blocks(nbins, 1024);
   hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, b.cnt, nbins, b.delim, b.sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, b.sums, nsb, b.misc);
   hipLaunchKernelGGL(k_scan_add, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sums,
                      b.misc, b.delim + nbins);
-  hipLaunchKernelGGL(k_copy_u32, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.cursor);
-  hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, b.bins, n, b.cursor, b.sorted);
+  hipLaunchKernelGGL(k_scatter_ranked, dim3(blocks(n, 256)), dim3(256), 0, s, b.bins, b.scratch, n, b.delim,
+                     b.sorted);
   hipLaunchKernelGGL(k_sort_cells, dim3(blocks(nbins, 256)), dim3(256), 0, s, b.delim, nbins, b.sorted,
                      b.big, b.misc + 1, key);
   hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(256), 0, s, b.delim, b.big, b.misc + 1, b.sorted,
@@ -435,7 +489,7 @@ hipError_t launch_grid_order(const double* x, const double* y, const double* z, 
   hipError_t e;
   if ((e = launch_fill_u32(b.cnt, (size_t)nbins, 0u, s)) != hipSuccess) return e;
   if ((e = launch_fill_u32(b.misc, 4, 0u, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt);
+  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, z, n, g, b.bins, b.cnt, nullptr);
   const int nsb = blocks(nbins, 1024);
   hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, b.cnt, nbins, b.delim, b.sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, b.sums, nsb, b.misc);

@@ -13,9 +13,9 @@ struct SortBufs {
   uint32_t* bins;     // [n]
   uint32_t* cnt;      // [nbins]
   uint32_t* delim;    // [nbins + 1]
-  uint32_t* cursor;   // [nbins]
+  uint32_t* cursor;   // [nbins] (grid-order binning only)
   uint32_t* sorted;   // [n]
-  uint32_t* scratch;  // [n]
+  uint32_t* scratch;  // [n]: slots in the cell ranges, then the in-cell sort's copy
   uint32_t* sums;     // [ceil(nbins / 1024) + 1]
   uint32_t* big;      // [nbins]
   uint32_t* misc;     // [4]: total, nbig
