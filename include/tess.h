@@ -407,7 +407,7 @@ tess_status tess_batch_get_config(const tess_batch* b, tess_batch_config* cfg);
 tess_status tess_batch_compute(tess_batch* b, const tess_model* m);
 /* Time the candidate configurations on this batch and keep the fastest:
  * first_capacity 64 or 128 x xcd_chunk 0, 256 or 4096, and with a model the
- * learned order at capacity 64 with xcd_chunk 0 or 256 (redo_chunk 4).
+ * learned order at capacity 64 with xcd_chunk 0 or 256 (redo_chunk 1).
  * cache_path (nullable): a text file of earlier results keyed by device and
  * batch shape; a hit skips the timing, a miss appends a line. */
 tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cache_path, tess_batch_config* best);

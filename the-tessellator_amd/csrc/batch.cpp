@@ -65,7 +65,7 @@ struct Mail {
 struct tess_batch {
   tess_ctx* ctx = nullptr;
   tess_opts opts;
-  tess_batch_config cfg{64, 0, 4, 0, 0};  // capacity 64, interleaved, chunk 4, list order, graphs
+  tess_batch_config cfg{64, 0, 1, 0, 0};  // capacity 64, interleaved, one rerun cell per grab, list order, graphs
   size_t nsets = 0, N = 0;
   std::vector<size_t> n, pbase;                        // points per set, first point
   std::vector<size_t> bbase, cbase, sbase, partbase;  // delim, bins, scan sums, partial bounds
@@ -762,9 +762,9 @@ tess_status tess_batch_tune(tess_batch* b, const tess_model* m, const char* cach
   }
   std::vector<tess_batch_config> cand;
   for (int cap : {64, 128})
-    for (int xcd : {0, 256, 4096}) cand.push_back({cap, xcd, 4, 0, 0});
+    for (int xcd : {0, 256, 4096}) cand.push_back({cap, xcd, 1, 0, 0});
   if (can_learn)
-    for (int xcd : {0, 256}) cand.push_back({64, xcd, 4, 1, 0});
+    for (int xcd : {0, 256}) cand.push_back({64, xcd, 1, 1, 0});
   double best_ms = 1e300;
   tess_batch_config bk = b->cfg;
   for (const tess_batch_config& k : cand) {
