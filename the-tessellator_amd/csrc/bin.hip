@@ -299,32 +299,46 @@ __global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ d
                                                   uint32_t* __restrict__ sorted,
                                                   uint32_t* __restrict__ scratch,
                                                   const uint32_t* __restrict__ key) {
-  __shared__ uint32_t sid[kSortLds];
+  __shared__ __align__(16) uint32_t sid[kSortLds];
   __shared__ uint32_t skey[kSortLds];
   const uint32_t nb = *nbig;
   for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
     const uint32_t b = big[q];
     const uint32_t s = delim[b], e = delim[b + 1], m = e - s;
     if (m <= kSortLds) {
-      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        const uint32_t v = sorted[s + i];
+      const uint32_t m4 = (m + 3) >> 2;
+      for (uint32_t i = threadIdx.x; i < 4 * m4; i += blockDim.x) {
+        // past the cell: the largest id, which no id of the cell exceeds
+        const uint32_t v = i < m ? sorted[s + i] : 0xffffffffu;
         sid[i] = v;
-        if (key) skey[i] = key[v];
+        if (key && i < m) skey[i] = key[v];
       }
       __syncthreads();
-      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        const uint32_t v = sid[i];
-        uint32_t r = 0;
-        if (key) {
-          const uint32_t kv = skey[i];
+      if (key) {
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+          const uint32_t v = sid[i], kv = skey[i];
+          uint32_t r = 0;
           for (uint32_t j = 0; j < m; ++j) {
             const uint32_t kj = skey[j];
             r += (kj < kv || (kj == kv && sid[j] < v)) ? 1u : 0u;
           }
-        } else {
-          for (uint32_t j = 0; j < m; ++j) r += sid[j] < v ? 1u : 0u;
+          sorted[s + r] = v;
         }
-        sorted[s + r] = v;
+      } else {
+        // two elements per thread and pass, four staged ids per LDS read
+        const uint4* s4 = reinterpret_cast<const uint4*>(sid);
+        for (uint32_t i = threadIdx.x; i < m; i += 2 * blockDim.x) {
+          const uint32_t i1 = i + blockDim.x;
+          const uint32_t v0 = sid[i], v1 = i1 < m ? sid[i1] : 0u;
+          uint32_t r0 = 0, r1 = 0;
+          for (uint32_t j = 0; j < m4; ++j) {
+            const uint4 w = s4[j];
+            r0 += (w.x < v0 ? 1u : 0u) + (w.y < v0 ? 1u : 0u) + (w.z < v0 ? 1u : 0u) + (w.w < v0 ? 1u : 0u);
+            r1 += (w.x < v1 ? 1u : 0u) + (w.y < v1 ? 1u : 0u) + (w.z < v1 ? 1u : 0u) + (w.w < v1 ? 1u : 0u);
+          }
+          sorted[s + r0] = v0;
+          if (i1 < m) sorted[s + r1] = v1;
+        }
       }
       __syncthreads();
       continue;
