@@ -18,6 +18,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -113,6 +114,70 @@ int64_t icbrt_floor(double x) {
 }  // namespace tess
 
 namespace tess {
+
+namespace {
+
+// dst <- src on up to eight host threads, in pieces of at least 4 MiB.
+void host_copy(char* dst, const char* src, size_t len) {
+  const size_t kPiece = size_t(4) << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t t = std::min<size_t>({size_t(8), size_t(hw), (len + kPiece - 1) / kPiece});
+  if (t <= 1) {
+    std::memcpy(dst, src, len);
+    return;
+  }
+  const size_t per = (len + t - 1) / t;
+  std::vector<std::thread> th;
+  th.reserve(t - 1);
+  size_t k = 1;
+  for (; k < t && k * per < len; ++k) {
+    const size_t o = k * per, l = std::min(per, len - o);
+    try {
+      th.emplace_back([=] { std::memcpy(dst + o, src + o, l); });
+    } catch (...) {  // no thread: this piece and the rest on this one
+      break;
+    }
+  }
+  for (size_t o = k * per; o < len; o += per) std::memcpy(dst + o, src + o, std::min(per, len - o));
+  std::memcpy(dst, src, std::min(per, len));
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+hipError_t download(tess_ctx* c, void* dst, const void* src, size_t bytes) {
+  constexpr size_t kChunk = Staging::kBytes;
+  if (bytes < 2 * kChunk) return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  Staging& S = c->stage;
+  std::lock_guard<std::mutex> g(S.mu);
+  for (int k = 0; k < 2; ++k) {
+    if (!S.buf[k] && hipHostMalloc(&S.buf[k], kChunk, hipHostMallocDefault) != hipSuccess) {
+      S.buf[k] = nullptr;
+      return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);  // no pinned memory: the plain copy
+    }
+    if (!S.ev[k] && hipEventCreateWithFlags(&S.ev[k], hipEventDisableTiming) != hipSuccess) {
+      S.ev[k] = nullptr;
+      return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    }
+  }
+  const size_t nch = (bytes + kChunk - 1) / kChunk;
+  auto len_of = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
+  // chunk i goes to buffer i & 1; chunk i + 1 is issued before chunk i is
+  // copied out, into the buffer chunk i - 1 left (copied out already)
+  auto issue = [&](size_t i) {
+    hipError_t e = hipMemcpyAsync(S.buf[i & 1], static_cast<const char*>(src) + i * kChunk, len_of(i),
+                                  hipMemcpyDeviceToHost, c->stream);
+    return e == hipSuccess ? hipEventRecord(S.ev[i & 1], c->stream) : e;
+  };
+  hipError_t e = issue(0);
+  for (size_t i = 0; i < nch && e == hipSuccess; ++i) {
+    if (i + 1 < nch && (e = issue(i + 1)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(S.ev[i & 1])) != hipSuccess) break;
+    host_copy(static_cast<char*>(dst) + i * kChunk, static_cast<const char*>(S.buf[i & 1]), len_of(i));
+  }
+  if (e != hipSuccess) (void)hipStreamSynchronize(c->stream);  // nothing left in flight into the staging
+  return e;
+}
 
 tess_status alloc_all(tess_diagram* d) {
   tess_ctx* c = d->ctx;
@@ -614,13 +679,11 @@ tess_status tess_diagram_copy_cells(const tess_diagram* d, double* volume, uint3
   (void)hipSetDevice(c->device);
   const size_t n = d->n;
   hipError_t e = hipSuccess;
-  if (volume && e == hipSuccess) e = hipMemcpy(volume, d->volume.p, 8 * n, hipMemcpyDeviceToHost);
-  if (fc && e == hipSuccess) e = hipMemcpy(fc, d->fcount.p, 4 * n, hipMemcpyDeviceToHost);
-  if (fo && e == hipSuccess) e = hipMemcpy(fo, d->foff.p, 8 * (n + 1), hipMemcpyDeviceToHost);
-  if (nbr && e == hipSuccess && d->total_faces)
-    e = hipMemcpy(nbr, d->nbr.p, 4 * d->total_faces, hipMemcpyDeviceToHost);
-  if (area && e == hipSuccess && d->total_faces)
-    e = hipMemcpy(area, d->area.p, 8 * d->total_faces, hipMemcpyDeviceToHost);
+  if (volume && e == hipSuccess) e = download(c, volume, d->volume.p, 8 * n);
+  if (fc && e == hipSuccess) e = download(c, fc, d->fcount.p, 4 * n);
+  if (fo && e == hipSuccess) e = download(c, fo, d->foff.p, 8 * (n + 1));
+  if (nbr && e == hipSuccess && d->total_faces) e = download(c, nbr, d->nbr.p, 4 * d->total_faces);
+  if (area && e == hipSuccess && d->total_faces) e = download(c, area, d->area.p, 8 * d->total_faces);
   return c->hip(e, "download");
 }
 
@@ -642,8 +705,7 @@ tess_status tess_diagram_copy_stats(const tess_diagram* d, tess_cell_stats* stat
   if (!d->computed || !d->opts.collect_stats)
     return d->ctx->fail(TESS_E_STATE, "stats need collect_stats = 1 and a computed diagram");
   (void)hipSetDevice(d->ctx->device);
-  return d->ctx->hip(hipMemcpy(stats, d->stats.p, sizeof(tess_cell_stats) * d->n, hipMemcpyDeviceToHost),
-                     "download");
+  return d->ctx->hip(download(d->ctx, stats, d->stats.p, sizeof(tess_cell_stats) * d->n), "download");
 }
 
 tess_status tess_diagram_timings(const tess_diagram* d, tess_timings* out) {

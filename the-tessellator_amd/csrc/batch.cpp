@@ -816,12 +816,12 @@ tess_status tess_batch_copy_cells(const tess_batch* b, size_t set, double* volum
   std::vector<uint64_t> off(n + 1);
   hipError_t e = hipMemcpy(off.data(), b->foff.p + p, 8 * (n + 1), hipMemcpyDeviceToHost);
   const uint64_t f0 = off[0], nf = off[n] - off[0];
-  if (e == hipSuccess && volume) e = hipMemcpy(volume, b->volume.p + p, 8 * n, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && fc) e = hipMemcpy(fc, b->fcount.p + p, 4 * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && volume) e = download(c, volume, b->volume.p + p, 8 * n);
+  if (e == hipSuccess && fc) e = download(c, fc, b->fcount.p + p, 4 * n);
   if (e == hipSuccess && fo)
     for (size_t i = 0; i <= n; ++i) fo[i] = off[i] - f0;
-  if (e == hipSuccess && nbr && nf) e = hipMemcpy(nbr, b->nbr.p + f0, 4 * nf, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && area && nf) e = hipMemcpy(area, b->area.p + f0, 8 * nf, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && nbr && nf) e = download(c, nbr, b->nbr.p + f0, 4 * nf);
+  if (e == hipSuccess && area && nf) e = download(c, area, b->area.p + f0, 8 * nf);
   return c->hip(e, "download");
 }
 

@@ -200,12 +200,10 @@ tess_status tess_diagram_copy_geometry(const tess_diagram* d, uint64_t* vertex_o
   if (!d->geo_ready) return c->fail(TESS_E_STATE, "geometry read before tess_diagram_compute_geometry");
   (void)hipSetDevice(c->device);
   hipError_t e = hipSuccess;
-  if (vertex_offset) e = hipMemcpy(vertex_offset, d->geo_voff.p, 8 * (d->n + 1), hipMemcpyDeviceToHost);
-  if (vertices && d->geo_nv && e == hipSuccess)
-    e = hipMemcpy(vertices, d->geo_v.p, 24 * d->geo_nv, hipMemcpyDeviceToHost);
-  if (ring_offset && e == hipSuccess)
-    e = hipMemcpy(ring_offset, d->geo_roff.p, 8 * (d->total_faces + 1), hipMemcpyDeviceToHost);
-  if (ring && d->geo_nr && e == hipSuccess) e = hipMemcpy(ring, d->geo_ring.p, 2 * d->geo_nr, hipMemcpyDeviceToHost);
+  if (vertex_offset) e = download(c, vertex_offset, d->geo_voff.p, 8 * (d->n + 1));
+  if (vertices && d->geo_nv && e == hipSuccess) e = download(c, vertices, d->geo_v.p, 24 * d->geo_nv);
+  if (ring_offset && e == hipSuccess) e = download(c, ring_offset, d->geo_roff.p, 8 * (d->total_faces + 1));
+  if (ring && d->geo_nr && e == hipSuccess) e = download(c, ring, d->geo_ring.p, 2 * d->geo_nr);
   return c->hip(e, "download");
 }
 

@@ -210,12 +210,12 @@ tess_status tess_site_cells_copy(const tess_site_cells* p, double* volume, uint3
   if (!p) return TESS_E_INVALID;
   (void)hipSetDevice(p->ctx->device);
   hipError_t e = hipSuccess;
-  if (volume) e = hipMemcpy(volume, p->volume.p, 8 * p->m, hipMemcpyDeviceToHost);
-  if (face_count && e == hipSuccess) e = hipMemcpy(face_count, p->fcount.p, 4 * p->m, hipMemcpyDeviceToHost);
-  if (face_offset && e == hipSuccess) e = hipMemcpy(face_offset, p->foff.p, 8 * (p->m + 1), hipMemcpyDeviceToHost);
-  if (neighbor && p->total && e == hipSuccess)
-    e = hipMemcpy(neighbor, p->nbr.p, 4 * p->total, hipMemcpyDeviceToHost);
-  if (face_area && p->total && e == hipSuccess) e = hipMemcpy(face_area, p->area.p, 8 * p->total, hipMemcpyDeviceToHost);
+  tess_ctx* c = p->ctx;
+  if (volume) e = download(c, volume, p->volume.p, 8 * p->m);
+  if (face_count && e == hipSuccess) e = download(c, face_count, p->fcount.p, 4 * p->m);
+  if (face_offset && e == hipSuccess) e = download(c, face_offset, p->foff.p, 8 * (p->m + 1));
+  if (neighbor && p->total && e == hipSuccess) e = download(c, neighbor, p->nbr.p, 4 * p->total);
+  if (face_area && p->total && e == hipSuccess) e = download(c, face_area, p->area.p, 8 * p->total);
   return p->ctx->hip(e, "download");
 }
 

@@ -254,8 +254,8 @@ tess_status tess_query_copy(const tess_query* q, uint64_t* offsets, uint32_t* id
   if (!q) return TESS_E_INVALID;
   (void)hipSetDevice(q->ctx->device);
   hipError_t e = hipSuccess;
-  if (offsets) e = hipMemcpy(offsets, q->offsets.p, 8 * (q->nq + 1), hipMemcpyDeviceToHost);
-  if (ids && q->total && e == hipSuccess) e = hipMemcpy(ids, q->ids.p, 4 * q->total, hipMemcpyDeviceToHost);
+  if (offsets) e = download(q->ctx, offsets, q->offsets.p, 8 * (q->nq + 1));
+  if (ids && q->total && e == hipSuccess) e = download(q->ctx, ids, q->ids.p, 4 * q->total);
   return q->ctx->hip(e, "download");
 }
 

@@ -69,6 +69,25 @@ double shell_key(int ox, int oy, int oz, double sx, double sy, double sz);
 uint64_t bits(double v);
 int64_t icbrt_floor(double x);
 
+// Pinned host staging for large downloads into caller memory: the results
+// are copied into the two buffers in turn while host threads move the
+// previous chunk into the caller's (pageable) memory.
+struct Staging {
+  static constexpr size_t kBytes = size_t(64) << 20;  // per buffer
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::mutex mu;
+  Staging() = default;
+  Staging(const Staging&) = delete;
+  Staging& operator=(const Staging&) = delete;
+  ~Staging() {
+    for (int k = 0; k < 2; ++k) {
+      if (buf[k]) (void)hipHostFree(buf[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+  }
+};
+
 }  // namespace tess
 
 using namespace tess;
@@ -89,6 +108,7 @@ struct tess_ctx {
   DevBuf<uint32_t> fin_list, fin_ctr;
   DevBuf<uint32_t> fin_tab, fin_cap;
   DevBuf<uint64_t> fin_tab64, fin_cap64;
+  Staging stage;  // large downloads (tess::download)
 
   tess_status fail(tess_status s, const std::string& m) {
     err = m;
@@ -242,6 +262,12 @@ struct tess_model {
 };
 
 namespace tess {
+
+// dst <- src (device), bytes of it, ordered after the context stream's work
+// and finished on return.  Large copies go through the context's pinned
+// staging, chunk by chunk, overlapping the transfer of one chunk with the
+// host copy of the one before; small ones are one hipMemcpy.
+hipError_t download(tess_ctx* c, void* dst, const void* src, size_t bytes);
 
 tess_status alloc_all(tess_diagram* d);
 // bounds -> counting sort -> gather, enqueued only (graph-capturable)
