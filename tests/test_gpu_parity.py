@@ -115,6 +115,33 @@ def test_uniform_cells_match_oracle(ctx, n, seed, density):
         assert np.array_equal(st[k].astype(np.uint64), rs[k].astype(np.uint64)), k
 
 
+def test_thin_slab_box_matches_oracle(ctx):
+    """A box 250 times longer than it is thin ([0, 50] x [0, 1] x [0, 0.2]):
+    grid cells of 3.3 x 0.067 x 0.013, so the shell order's keys weigh the
+    three axes very differently and the security balls span many grid cells
+    along the thin axis.  Bitwise against the oracle, counters included, on
+    the single-set path and in a batch beside a unit-cube set."""
+    lo, hi = np.zeros(3), np.array([50.0, 1.0, 0.2])
+    pts = tess.uniform(20_000, 41, lo, hi)
+    d = tess.Diagram(ctx, pts, lo, hi, density=5.0, collect_stats=True)
+    d.compute()
+    got = d.cells()
+    ref = orc.compute(pts, lo, hi, density=5.0)
+    for k in ("face_count", "face_offset", "neighbor", "volume", "area"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert abs(got["volume"].sum() - 10.0) <= 1e-10 * 10.0
+    st, rs = d.stats(), ref["stats"]
+    for k in ("streamed", "classified", "cuts", "entries", "flops", "incident_cuts"):
+        assert np.array_equal(st[k].astype(np.uint64), rs[k].astype(np.uint64)), k
+    b = tess.Batch(ctx, [(tess.uniform(5_000, 42), UNIT_LO, UNIT_HI), (pts, lo, hi)], density=5.0)
+    b.compute()
+    got_b = b.cells(1)
+    for k in got:
+        assert np.array_equal(got_b[k], got[k]), k
+    b.close()
+    d.close()
+
+
 def test_incident_only_cuts_counted(ctx):
     """Cuts bordered by Incident vertices only (the reference's
     find_outgoing_edge skips them, polyhedron.rs:418-432): counted per cell
