@@ -156,3 +156,43 @@ def test_full_size_uniform_1m(ctx):
     sub["neighbor"] = np.concatenate([nb[fo[i]:fo[i + 1]] for i in sample])
     sub["area"] = np.concatenate([ar[fo[i]:fo[i + 1]] for i in sample])
     same_cells(sub, ref, rel=0.0)
+
+
+
+def test_large_downloads_staged_and_concurrent(ctx):
+    """Results above the staging threshold (128 MB) come down through the
+    context's pinned double buffer, chunk by chunk: every shared face has the
+    same area and neighbour pair from both sides and the volumes sum to the
+    box, and two host threads downloading from one context at the same time
+    (the staging is taken in turn) get the same arrays."""
+    import threading
+    n = 2_000_000
+    pts = tess.uniform(n, 9)
+    d = tess.Diagram(ctx, pts, LO, HI, density=5.0)
+    d.compute()
+    first = d.cells()
+    assert first["area"].nbytes > (128 << 20) and first["neighbor"].nbytes > (128 << 20) // 2
+    assert abs(first["volume"].sum() - 1) < 1e-10
+    fo = first["face_offset"].astype(np.int64)
+    owner = np.repeat(np.arange(n, dtype=np.int64), np.diff(fo))
+    nb, ar = first["neighbor"].astype(np.int64), first["area"]
+    inner = nb >= 0
+    k1, k2 = owner[inner] * n + nb[inner], nb[inner] * n + owner[inner]
+    o1, o2 = np.argsort(k1), np.argsort(k2)
+    assert np.array_equal(k1[o1], k2[o2])
+    scale = np.maximum(ar[inner][o1], np.median(ar[inner]))
+    assert (np.abs(ar[inner][o1] - ar[inner][o2]) / scale).max() < 1e-9
+    out = [None, None]
+
+    def work(k):
+        out[k] = d.cells()
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k in range(2):
+        for key in first:
+            assert np.array_equal(out[k][key], first[key]), (k, key)
+    d.close()
