@@ -296,7 +296,10 @@ void tess_site_cells_destroy(tess_site_cells* p);
  * its grid cell is clamped as in celery.rs:269-282).  radius: nradius = 1
  * (one radius for all) or nq values.  target_group >= 0 needs
  * TESS_QUERY_CLOUD and a diagram with groups.  A NaN radius lists nothing
- * for the box kinds and every point for the cloud, as in the reference. */
+ * for the box kinds and every point for the cloud, as in the reference.
+ * The call's device temporaries (per-query counts, the grid order of large
+ * query sets, the filtered kinds' upper-bound slots: 4 B per candidate
+ * point) are kept by the diagram for its next queries and freed with it. */
 enum {
   TESS_QUERY_CELLS = 0,
   TESS_QUERY_CELL_RADIUS = 1,

@@ -101,8 +101,9 @@ tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, 
     if (!(t = c->table(d->cpd, G.sx, G.sy, G.sz, L, &st))) return st;
   }
   DevBuf<double> qd, rd;
-  DevBuf<uint32_t> cnt;
-  DevBuf<uint64_t> sums;
+  auto& T = d->qtmp;  // serialised by the context lock
+  DevBuf<uint32_t>& cnt = T.cnt;
+  DevBuf<uint64_t>& sums = T.sums;
   if (cnt.alloc(nq) != hipSuccess || sums.alloc((nq + 1023) / 1024 + 1) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   if (device_inputs) {
@@ -117,8 +118,9 @@ tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, 
   }
   // Large sets: take the queries in grid order of their positions (the
   // results do not depend on the order the wavefronts take them in).
-  DevBuf<double> ox, oy, oz;
-  DevBuf<uint32_t> obins, ocnt, odelim, ocursor, oorder, osums, omisc;
+  DevBuf<double>&ox = T.ox, &oy = T.oy, &oz = T.oz;
+  DevBuf<uint32_t>&obins = T.obins, &ocnt = T.ocnt, &odelim = T.odelim, &ocursor = T.ocursor, &oorder = T.oorder,
+                   &osums = T.osums, &omisc = T.omisc;
   const bool ordered = nq >= kQueryOrderMin;
   if (ordered) {
     const size_t nb = (size_t)d->nbins;
@@ -162,7 +164,7 @@ tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, 
     QueryArgs b = a;
     b.kind = kind == TESS_QUERY_REAL_RADIUS ? TESS_QUERY_CELL_RADIUS : TESS_QUERY_CLOUD;
     b.groups = nullptr;
-    DevBuf<uint64_t> boff;
+    DevBuf<uint64_t>& boff = T.boff;
     uint64_t bound = 0;
     if (boff.alloc(nq + 1) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
     if ((st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK ||
@@ -172,7 +174,7 @@ tess_status query_impl(tess_diagram* d, int kind, const double* xyz, size_t nq, 
         (st = c->hip(hipMemcpyAsync(&bound, boff.p + nq, 8, hipMemcpyDeviceToHost, s), "download")) != TESS_OK ||
         (st = c->hip(hipStreamSynchronize(s), "query bound")) != TESS_OK)
       return st;
-    DevBuf<uint32_t> slots;
+    DevBuf<uint32_t>& slots = T.slots;
     if (bound <= kBoundMax && slots.alloc(bound) == hipSuccess) {
       a.offsets = boff.p;
       a.ids = slots.p;

@@ -231,6 +231,14 @@ struct tess_diagram {
   DevBuf<double> one_site, one_vol, one_area;
   DevBuf<uint32_t> one_u32;  // face count, status
   DevBuf<int32_t> one_nbr;
+  // temporaries of the radius queries, kept between calls (grown on demand,
+  // freed with the diagram): the per-query counts and their scans, the
+  // grid order of large query sets, the kept ids' upper-bound slots
+  struct {
+    DevBuf<uint32_t> cnt, obins, ocnt, odelim, ocursor, oorder, osums, omisc, slots;
+    DevBuf<uint64_t> sums, boff;
+    DevBuf<double> ox, oy, oz;
+  } qtmp;
   // geometry of every cell (tess_diagram_compute_geometry)
   DevBuf<uint32_t> geo_count;
   DevBuf<uint64_t> geo_voff, geo_rbase, geo_roff, geo_sums;
