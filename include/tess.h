@@ -258,7 +258,9 @@ tess_status tess_diagram_copy_geometry(const tess_diagram* d, uint64_t* vertex_o
  * as the cell's own).  search_radius: NaN = None, else the
  * expand_all_in_radius cut-off of tess_cell_at_index.  Sites must lie in the
  * container box (TESS_E_INVALID otherwise).  Results are laid out as
- * tess_diagram_copy_cells, indexed by site. */
+ * tess_diagram_copy_cells, indexed by site.  The call's device temporaries
+ * (the sites, a face arena of 64 faces per site, work lists) are kept by the
+ * diagram for its next calls and freed with it. */
 typedef struct tess_site_cells tess_site_cells;
 tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, size_t m, double search_radius,
                                             int64_t target_group, tess_site_cells** out);

@@ -11,6 +11,7 @@
 #include "launch.h"
 #include "runtime.h"
 
+// The results; the call's temporaries belong to the diagram (stmp).
 struct tess_site_cells {
   tess_ctx* ctx = nullptr;
   size_t m = 0;
@@ -18,13 +19,11 @@ struct tess_site_cells {
   uint32_t reruns = 0;
   double ms[2] = {0, 0};  // cell passes, face offsets + compaction
   bool have_stats = false;
-  DevBuf<double> sites, volume, sarea, area;
-  DevBuf<uint32_t> fcount, status, redo, redo_list, tredo, spill;
-  DevBuf<int32_t> snbr, nbr;  // face arena, CSR
-  DevBuf<uint64_t> foff, fsums, fpos;
-  DevBuf<unsigned long long> arena;
+  DevBuf<double> volume, area;
+  DevBuf<uint32_t> fcount, status;
+  DevBuf<int32_t> nbr;  // CSR
+  DevBuf<uint64_t> foff;
   DevBuf<tess_cell_stats> stats;
-  DevBuf<Summary> summary;
 };
 
 namespace {
@@ -76,33 +75,33 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
   p->ctx = c;
   p->m = m;
   p->have_stats = d->opts.collect_stats != 0;
-  if (p->sites.alloc(3 * m) != hipSuccess || p->volume.alloc(m) != hipSuccess || p->fcount.alloc(m) != hipSuccess ||
-      p->status.alloc(m) != hipSuccess || p->snbr.alloc(arena_capacity(m)) != hipSuccess ||
-      p->sarea.alloc(arena_capacity(m)) != hipSuccess || p->foff.alloc(m + 1) != hipSuccess ||
-      p->fsums.alloc((m + 1023) / 1024 + 1) != hipSuccess || p->summary.alloc(1) != hipSuccess ||
-      p->redo.alloc(m + 1) != hipSuccess || p->tredo.alloc(m + 1) != hipSuccess || p->spill.alloc(m + 1) != hipSuccess ||
-      p->fpos.alloc(m) != hipSuccess || p->arena.alloc(kArenaWords) != hipSuccess ||
-      (p->have_stats && p->stats.alloc(m) != hipSuccess))
+  auto& T = d->stmp;  // serialised by the context lock
+  if (p->volume.alloc(m) != hipSuccess || p->fcount.alloc(m) != hipSuccess || p->status.alloc(m) != hipSuccess ||
+      p->foff.alloc(m + 1) != hipSuccess || (p->have_stats && p->stats.alloc(m) != hipSuccess) ||
+      T.sites.alloc(3 * m) != hipSuccess || T.snbr.alloc(arena_capacity(m)) != hipSuccess ||
+      T.sarea.alloc(arena_capacity(m)) != hipSuccess || T.fsums.alloc((m + 1023) / 1024 + 1) != hipSuccess ||
+      T.summary.alloc(1) != hipSuccess || T.redo.alloc(m + 1) != hipSuccess || T.tredo.alloc(m + 1) != hipSuccess ||
+      T.spill.alloc(m + 1) != hipSuccess || T.fpos.alloc(m) != hipSuccess || T.arena.alloc(kArenaWords) != hipSuccess)
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   Events ev;
-  if ((st = c->hip(hipMemcpyAsync(p->sites.p, xyz, 24 * m, hipMemcpyHostToDevice, s), "upload")) != TESS_OK ||
-      (st = c->hip(hipMemsetAsync(p->summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32(p->redo.p, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32(p->tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32(p->spill.p, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(launch_fill_u32((uint32_t*)p->arena.p, 2 * kArenaWords, 0u, s), "fill")) != TESS_OK ||
+  if ((st = c->hip(hipMemcpyAsync(T.sites.p, xyz, 24 * m, hipMemcpyHostToDevice, s), "upload")) != TESS_OK ||
+      (st = c->hip(hipMemsetAsync(T.summary.p, 0, sizeof(Summary), s), "memset")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(T.redo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(T.tredo.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32(T.spill.p, 1, 0u, s), "fill")) != TESS_OK ||
+      (st = c->hip(launch_fill_u32((uint32_t*)T.arena.p, 2 * kArenaWords, 0u, s), "fill")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[0], s), "event")) != TESS_OK)
     return st;
   CellArgs a = base_args(d, t);
-  a.ext = p->sites.p;
+  a.ext = T.sites.p;
   a.work = nullptr;
   a.nwork = (int)m;
   a.xcd_chunk = 0;
   // larger sets: first pass in grid order of the sites, each XCD taking runs
   // of them, so that waves running together read the same grid cells (the
   // outputs stay per site); small sets are not worth the sort's launches
-  DevBuf<uint32_t> keys, keys_out, iota, order;
-  DevBuf<uint8_t> tmp;
+  DevBuf<uint32_t>&keys = T.keys, &keys_out = T.keys_out, &iota = T.iota, &order = T.order;
+  DevBuf<uint8_t>& tmp = T.sort_tmp;
   if (m >= kSiteOrderMin) {
     size_t tmp_bytes = 0;
     if ((st = c->hip(launch_site_order(nullptr, (int)m, a.grid, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -112,7 +111,7 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
     if (keys.alloc(m) != hipSuccess || keys_out.alloc(m) != hipSuccess || iota.alloc(m) != hipSuccess ||
         order.alloc(m) != hipSuccess || tmp.alloc(tmp_bytes) != hipSuccess)
       return c->fail(TESS_E_NOMEM, "device allocation failed");
-    if ((st = c->hip(launch_site_order(p->sites.p, (int)m, a.grid, keys.p, keys_out.p, iota.p, order.p, tmp.p,
+    if ((st = c->hip(launch_site_order(T.sites.p, (int)m, a.grid, keys.p, keys_out.p, iota.p, order.p, tmp.p,
                                        &tmp_bytes, s),
                      "sort")) != TESS_OK)
       return st;
@@ -123,53 +122,53 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
   a.radius_limit = search_radius;
   a.volume = p->volume.p;
   a.fcount = p->fcount.p;
-  a.snbr = p->snbr.p;
-  a.sarea = p->sarea.p;
-  a.fpos = p->fpos.p;
-  a.arena_next = p->arena.p;
+  a.snbr = T.snbr.p;
+  a.sarea = T.sarea.p;
+  a.fpos = T.fpos.p;
+  a.arena_next = T.arena.p;
   a.arena_cap = arena_capacity(m);
   a.status = p->status.p;
   a.stats = p->have_stats ? p->stats.p : nullptr;
-  a.summary = p->summary.p;
+  a.summary = T.summary.p;
   const int vpl = pick_vpl(d);
   // capacity overflows below 256 vertices: the rerun list; at 256: the
   // spill kernel's; shell-table exhaustions: their own list
-  a.redo = (vpl < 4 ? p->redo.p : p->spill.p) + 1;
-  a.nredo = vpl < 4 ? p->redo.p : p->spill.p;
-  a.tredo = p->tredo.p + 1;
-  a.ntredo = p->tredo.p;
+  a.redo = (vpl < 4 ? T.redo.p : T.spill.p) + 1;
+  a.nredo = vpl < 4 ? T.redo.p : T.spill.p;
+  a.tredo = T.tredo.p + 1;
+  a.ntredo = T.tredo.p;
   a.defer_table = 1;
   const CellArgs a0 = a;
   if ((st = c->hip(launch_cells(vpl, a, s), "cell kernel")) != TESS_OK) return st;
   uint32_t nredo = 0;
-  if ((st = c->hip(hipMemcpyAsync(&nredo, p->redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
+  if ((st = c->hip(hipMemcpyAsync(&nredo, T.redo.p, 4, hipMemcpyDeviceToHost, s), "redo")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "cell kernel")) != TESS_OK)
     return st;
   if (nredo && vpl < 4) {  // sites that outgrew the first pass: the largest variant
-    if (p->redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
-    a.work = p->redo_list.p;
+    if (T.redo_list.alloc(nredo) != hipSuccess) return c->fail(TESS_E_NOMEM, "device allocation failed");
+    a.work = T.redo_list.p;
     a.nwork = (int)nredo;
     a.xcd_chunk = 0;
-    a.redo = p->spill.p + 1;
-    a.nredo = p->spill.p;
-    if ((st = c->hip(hipMemcpyAsync(p->redo_list.p, p->redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
+    a.redo = T.spill.p + 1;
+    a.nredo = T.spill.p;
+    if ((st = c->hip(hipMemcpyAsync(T.redo_list.p, T.redo.p + 1, 4ull * nredo, hipMemcpyDeviceToDevice, s),
                      "redo")) != TESS_OK ||
         (st = c->hip(launch_cells(4, a, s), "cell kernel")) != TESS_OK)
       return st;
   }
   // sites past the shell table or capacity 256
   std::vector<uint32_t> tab, spl;
-  if ((st = read_list(c, p->tredo.p, tab)) != TESS_OK || (st = read_list(c, p->spill.p, spl)) != TESS_OK) return st;
+  if ((st = read_list(c, T.tredo.p, tab)) != TESS_OK || (st = read_list(c, T.spill.p, spl)) != TESS_OK) return st;
   FinishCounts fin;
   if ((!tab.empty() || !spl.empty()) &&
       (st = finish_cells(d, a0, std::move(tab), std::move(spl), &fin)) != TESS_OK)
     return st;
   if (nredo && vpl < 4 &&
-      (st = c->hip(launch_mark_status(p->redo_list.p, (int)nredo, nullptr, p->status.p, a.stats, TESS_CELL_RERUN, s),
+      (st = c->hip(launch_mark_status(T.redo_list.p, (int)nredo, nullptr, p->status.p, a.stats, TESS_CELL_RERUN, s),
                    "mark")) != TESS_OK)
     return st;
   if ((st = c->hip(hipEventRecord(ev.e[1], s), "event")) != TESS_OK ||
-      (st = c->hip(launch_face_offsets(p->fcount.p, (int)m, p->foff.p, p->fsums.p, s), "face offsets")) != TESS_OK ||
+      (st = c->hip(launch_face_offsets(p->fcount.p, (int)m, p->foff.p, T.fsums.p, s), "face offsets")) != TESS_OK ||
       (st = c->hip(hipMemcpyAsync(&p->total, p->foff.p + m, 8, hipMemcpyDeviceToHost, s), "total")) != TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "face offsets")) != TESS_OK)
     return st;
@@ -177,11 +176,11 @@ tess_status tess_diagram_cells_at_particles(tess_diagram* d, const double* xyz, 
     return c->fail(TESS_E_NOMEM, "device allocation failed");
   Summary sm2;
   float ms[2] = {0, 0};
-  if ((st = c->hip(launch_compact(p->fcount.p, p->foff.p, (int)m, p->snbr.p, p->sarea.p, p->nbr.p, p->area.p, s,
-                                  p->fpos.p),
+  if ((st = c->hip(launch_compact(p->fcount.p, p->foff.p, (int)m, T.snbr.p, T.sarea.p, p->nbr.p, p->area.p, s,
+                                  T.fpos.p),
                    "compact")) != TESS_OK ||
       (st = c->hip(hipEventRecord(ev.e[2], s), "event")) != TESS_OK ||
-      (st = c->hip(hipMemcpyAsync(&sm2, p->summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
+      (st = c->hip(hipMemcpyAsync(&sm2, T.summary.p, sizeof(Summary), hipMemcpyDeviceToHost, s), "summary")) !=
           TESS_OK ||
       (st = c->hip(hipStreamSynchronize(s), "compact")) != TESS_OK ||
       (st = c->hip(hipEventElapsedTime(&ms[0], ev.e[0], ev.e[1]), "event")) != TESS_OK ||

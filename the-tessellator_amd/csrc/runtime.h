@@ -239,6 +239,19 @@ struct tess_diagram {
     DevBuf<uint64_t> sums, boff;
     DevBuf<double> ox, oy, oz;
   } qtmp;
+  // temporaries of tess_diagram_cells_at_particles, kept between calls
+  // (grown on demand, freed with the diagram): the uploaded sites, the face
+  // arena and its counters, the rerun / shell-table / spill lists, the grid
+  // order of the sites and the scan sums
+  struct {
+    DevBuf<double> sites, sarea;
+    DevBuf<int32_t> snbr;
+    DevBuf<uint64_t> fpos, fsums;
+    DevBuf<unsigned long long> arena;
+    DevBuf<uint32_t> redo, redo_list, tredo, spill, keys, keys_out, iota, order;
+    DevBuf<uint8_t> sort_tmp;
+    DevBuf<Summary> summary;
+  } stmp;
   // geometry of every cell (tess_diagram_compute_geometry)
   DevBuf<uint32_t> geo_count;
   DevBuf<uint64_t> geo_voff, geo_rbase, geo_roff, geo_sums;
