@@ -115,10 +115,8 @@ int64_t icbrt_floor(double x) {
 
 namespace tess {
 
-namespace {
-
-// dst <- src on up to eight host threads, in pieces of at least 4 MiB.
 void host_copy(char* dst, const char* src, size_t len) {
+  if (len == 0) return;  // (memcpy takes no null pointers, even for no bytes)
   const size_t kPiece = size_t(4) << 20;
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const size_t t = std::min<size_t>({size_t(8), size_t(hw), (len + kPiece - 1) / kPiece});
@@ -142,8 +140,6 @@ void host_copy(char* dst, const char* src, size_t len) {
   std::memcpy(dst, src, std::min(per, len));
   for (auto& x : th) x.join();
 }
-
-}  // namespace
 
 hipError_t download(tess_ctx* c, void* dst, const void* src, size_t bytes) {
   constexpr size_t kChunk = Staging::kBytes;

@@ -289,6 +289,9 @@ namespace tess {
 // staging, chunk by chunk, overlapping the transfer of one chunk with the
 // host copy of the one before; small ones are one hipMemcpy.
 hipError_t download(tess_ctx* c, void* dst, const void* src, size_t bytes);
+// dst <- src, len bytes, on up to eight host threads in pieces of at least
+// 4 MiB (the staged download's copy out of pinned memory)
+void host_copy(char* dst, const char* src, size_t len);
 
 tess_status alloc_all(tess_diagram* d);
 // bounds -> counting sort -> gather, enqueued only (graph-capturable)

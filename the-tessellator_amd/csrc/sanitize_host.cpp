@@ -2,8 +2,9 @@
 // UndefinedBehaviorSanitizer, on the CPU (no GPU needed): the shell-table
 // builder (celery.rs:418-679: every offset once, keys ascending, ties by
 // generation order, truncated tables a prefix of the full one), the exact
-// integer cube root of the grid size (celery.rs:161-162), and the C ABI's
-// argument checks.  Built with the library's host objects instrumented
+// integer cube root of the grid size (celery.rs:161-162), the threaded host
+// copy of the staged downloads (every byte once, none outside), and the C
+// ABI's argument checks.  Built with the library's host objects instrumented
 // (`make -C the-tessellator_amd sanitize`); exit status 0 = clean.
 #include <cmath>
 #include <cstdio>
@@ -68,6 +69,15 @@ int main() {
   for (size_t i = 0; i < sizeof(cubes) / sizeof(cubes[0]); ++i)
     expect(tess::icbrt_floor(cubes[i]) == roots[i], "integer cube root");
   expect(tess::icbrt_floor(0.5) == 0 && tess::icbrt_floor(std::nan("")) == 0, "cube root below 1 and of NaN");
+  // the staged download's host copy: exact-size buffers, so a byte written
+  // or read outside them is a sanitizer report
+  const size_t MiB = size_t(1) << 20;
+  for (size_t len : {size_t(0), size_t(1), 4 * MiB - 1, 4 * MiB, 4 * MiB + 1, 9 * MiB + 3, 33 * MiB + 7, 64 * MiB}) {
+    std::vector<char> src(len), dst(len, 0);
+    for (size_t i = 0; i < len; ++i) src[i] = (char)(i * 131 + (i >> 11));
+    tess::host_copy(dst.data(), src.data(), len);
+    expect(dst == src, "threaded host copy");
+  }
   // the C ABI refuses bad arguments without touching a device
   tess_opts o;
   tess_opts_default(&o);
