@@ -100,19 +100,10 @@ struct tess_batch {
   std::vector<TableKey> table_of;  // per set, as uploaded
   Graph gbin, gcells;
   std::vector<int64_t> gcells_key;  // config + slot counts the cells graph was captured for
-  // side streams the sets' binning is spread over (default priority), and
-  // the events that fork them from and join them back into the context's
-  // stream
-  struct Side {
-    std::vector<hipStream_t> s;
-    std::vector<hipEvent_t> join;
-    hipEvent_t fork = nullptr;
-    ~Side() {
-      for (hipStream_t x : s) (void)hipStreamDestroy(x);
-      for (hipEvent_t e : join) (void)hipEventDestroy(e);
-      if (fork) (void)hipEventDestroy(fork);
-    }
-  } side;
+  // every set's first cell and first delimiter over the batch (nsets + 1
+  // each) and its cells per dimension, for the binning of all sets at once
+  DevBuf<uint32_t> cbase_dev, bbase_dev, pblk_dev;  // + first bounds block of each set
+  DevBuf<int32_t> cpd_dev;
   uint64_t total_faces = 0;
   tess_timings tim;
   bool computed = false;
@@ -188,42 +179,43 @@ tess_status capture_end(tess_ctx* c, Graph& g) {
   return st;
 }
 
-// The binning of every set spread over side streams forked from the
-// context's stream and joined back into it (the sets' binning buffers are
-// disjoint slices, so their short kernel chains can run side by side; under
-// capture the fork and the joins become graph edges), then the check.
-constexpr size_t kBinStreams = 8;
-
+// The binning of every set in one launch per pass (the sets' grids,
+// histograms, delimiters and sorted indices land in their slices of the
+// combined arrays, as a set-by-set binning would leave them), then the
+// check.
 tess_status enqueue_binning(tess_batch* b) {
   tess_ctx* c = b->ctx;
   hipStream_t s = c->stream;
   tess_status st;
-  const size_t k = std::min(kBinStreams, b->nsets);
-  if (b->side.s.size() < k) {
-    if (!b->side.fork && (st = c->hip(hipEventCreateWithFlags(&b->side.fork, hipEventDisableTiming), "event")) != TESS_OK)
-      return st;
-    while (b->side.s.size() < k) {
-      hipStream_t x = nullptr;
-      hipEvent_t e = nullptr;
-      if ((st = c->hip(hipStreamCreateWithFlags(&x, hipStreamNonBlocking), "stream")) != TESS_OK) return st;
-      b->side.s.push_back(x);
-      if ((st = c->hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event")) != TESS_OK) return st;
-      b->side.join.push_back(e);
-    }
-  }
-  if ((st = c->hip(launch_fill_u32(b->ctrl.p + kCtrlBad, 1, 0u, s), "fill")) != TESS_OK ||
-      (st = c->hip(hipEventRecord(b->side.fork, s), "event")) != TESS_OK)
-    return st;
-  tess_status err = TESS_OK;
-  for (size_t j = 0; j < k && err == TESS_OK; ++j) err = c->hip(hipStreamWaitEvent(b->side.s[j], b->side.fork, 0), "fork");
-  for (size_t set = 0; set < b->nsets && err == TESS_OK; ++set) err = bin_enqueue(b->views[set].get(), b->side.s[set % k]);
-  // every side stream joins back, also after an error (a capture must end joined)
-  for (size_t j = 0; j < k; ++j) {
-    tess_status e1 = c->hip(hipEventRecord(b->side.join[j], b->side.s[j]), "join");
-    tess_status e2 = c->hip(hipStreamWaitEvent(s, b->side.join[j], 0), "join");
-    if (err == TESS_OK) err = e1 != TESS_OK ? e1 : e2;
-  }
-  if (err != TESS_OK) return err;
+  if ((st = c->hip(launch_fill_u32(b->ctrl.p + kCtrlBad, 1, 0u, s), "fill")) != TESS_OK) return st;
+  SegBinning a;
+  a.nsets = (int)b->nsets;
+  a.n = (uint32_t)b->N;
+  a.nb = (uint32_t)b->cnt.n;
+  a.pbase = b->base_dev.p;
+  a.cbase = b->cbase_dev.p;
+  a.bbase = b->bbase_dev.p;
+  a.cpd = b->cpd_dev.p;
+  a.pblk = b->pblk_dev.p;
+  a.nparts = (uint32_t)(b->part.n / 6);
+  a.part = b->part.p;
+  a.x = b->x.p;
+  a.y = b->y.p;
+  a.z = b->z.p;
+  a.grid = b->grid.p;
+  a.bins = b->bins.p;
+  a.rank = b->scratch.p;
+  a.cnt = b->cnt.p;
+  a.gscan = b->cursor.p;
+  a.sums = b->sums.p;
+  a.delim = b->delim.p;
+  a.sorted = b->sorted.p;
+  a.big = b->big.p;
+  a.misc = b->misc.p;
+  a.sx = b->sx.p;
+  a.sy = b->sy.p;
+  a.sz = b->sz.p;
+  if ((st = c->hip(launch_seg_binning(a, s), "binning")) != TESS_OK) return st;
   return c->hip(launch_batch_check(b->delim.p, b->dend_dev.p, b->base_dev.p, (int)b->nsets, b->ctrl.p + kCtrlBad, s),
                 "binning check");
 }
@@ -644,7 +636,7 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
     S += (nb + 1023) / 1024 + 1;
     P += 6 * std::min<size_t>(256, (n[s] + 255) / 256);
   }
-  if (N > (size_t)UINT32_MAX) return ctx->fail(TESS_E_INVALID, "batch too large");
+  if (N > (size_t)UINT32_MAX || B > (size_t)UINT32_MAX) return ctx->fail(TESS_E_INVALID, "batch too large");
   b->N = N;
   bool ok = b->x.alloc(N) == hipSuccess && b->y.alloc(N) == hipSuccess && b->z.alloc(N) == hipSuccess &&
             b->sx.alloc(N) == hipSuccess && b->sy.alloc(N) == hipSuccess && b->sz.alloc(N) == hipSuccess &&
@@ -657,6 +649,8 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
             b->foff.alloc(N + 1) == hipSuccess && b->fsums.alloc((N + 1023) / 1024 + 1) == hipSuccess &&
             b->items.alloc(N) == hipSuccess && b->redo.alloc(N) == hipSuccess && b->redo2.alloc(N) == hipSuccess &&
             b->base_dev.alloc(nsets + 1) == hipSuccess && b->dend_dev.alloc(nsets) == hipSuccess &&
+            b->cbase_dev.alloc(nsets + 1) == hipSuccess && b->bbase_dev.alloc(nsets + 1) == hipSuccess &&
+            b->cpd_dev.alloc(nsets) == hipSuccess && b->pblk_dev.alloc(nsets + 1) == hipSuccess &&
             b->summary.alloc(nsets) == hipSuccess && b->args.alloc(nsets) == hipSuccess &&
             b->ctrl.alloc(kCtrlWords) == hipSuccess && b->fpos.alloc(N) == hipSuccess &&
             b->tredo.alloc(N) == hipSuccess && b->spill.alloc(N) == hipSuccess && b->arena.alloc(kArenaWords) == hipSuccess &&
@@ -680,11 +674,28 @@ tess_status tess_batch_create(tess_ctx* ctx, size_t nsets, const double* const* 
   // base: first point of each set (and N); dend: index of each set's last
   // delimiter, which must equal the set's size after binning
   std::vector<uint64_t> base(nsets + 1), dend(nsets);
+  std::vector<uint32_t> cb(nsets + 1), bb(nsets + 1), pb(nsets + 1);
+  std::vector<int32_t> cpds(nsets);
   for (size_t s = 0; s < nsets; ++s) {
     base[s] = b->pbase[s];
     dend[s] = (s + 1 < nsets ? b->bbase[s + 1] : B) - 1;
+    cb[s] = (uint32_t)b->cbase[s];
+    bb[s] = (uint32_t)b->bbase[s];
+    pb[s] = (uint32_t)(b->partbase[s] / 6);
+    cpds[s] = (int32_t)(icbrt_floor((double)n[s] / b->opts.cell_density) + 1);
   }
   base[nsets] = N;
+  cb[nsets] = (uint32_t)C;
+  bb[nsets] = (uint32_t)B;
+  pb[nsets] = (uint32_t)(P / 6);
+  if ((st = ctx->hip(hipMemcpy(b->cbase_dev.p, cb.data(), 4 * cb.size(), hipMemcpyHostToDevice), "upload")) !=
+          TESS_OK ||
+      (st = ctx->hip(hipMemcpy(b->bbase_dev.p, bb.data(), 4 * bb.size(), hipMemcpyHostToDevice), "upload")) !=
+          TESS_OK ||
+      (st = ctx->hip(hipMemcpy(b->cpd_dev.p, cpds.data(), 4 * cpds.size(), hipMemcpyHostToDevice), "upload")) !=
+          TESS_OK ||
+      (st = ctx->hip(hipMemcpy(b->pblk_dev.p, pb.data(), 4 * pb.size(), hipMemcpyHostToDevice), "upload")) != TESS_OK)
+    return st;
   if ((st = ctx->hip(hipMemcpy(b->base_dev.p, base.data(), 8 * base.size(), hipMemcpyHostToDevice), "upload")) !=
           TESS_OK ||
       (st = ctx->hip(hipMemcpy(b->dend_dev.p, dend.data(), 8 * dend.size(), hipMemcpyHostToDevice), "upload")) !=

@@ -293,6 +293,58 @@ __global__ void k_sort_cells(const uint32_t* __restrict__ delim, int nbins,
 // element written straight to its rank; a larger one is ranked against
 // global memory into a scratch copy, then copied back.
 constexpr uint32_t kSortLds = 4096;
+__device__ void rank_sort_cell(uint32_t* __restrict__ sorted, uint32_t* __restrict__ scratch, uint32_t s,
+                               uint32_t e, const uint32_t* __restrict__ key, uint32_t* sid, uint32_t* skey) {
+  const uint32_t m = e - s;
+  if (m <= kSortLds) {
+    const uint32_t m4 = (m + 3) >> 2;
+    for (uint32_t i = threadIdx.x; i < 4 * m4; i += blockDim.x) {
+      // past the cell: the largest id, which no id of the cell exceeds
+      const uint32_t v = i < m ? sorted[s + i] : 0xffffffffu;
+      sid[i] = v;
+      if (key && i < m) skey[i] = key[v];
+    }
+    __syncthreads();
+    if (key) {
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t v = sid[i], kv = skey[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < m; ++j) {
+          const uint32_t kj = skey[j];
+          r += (kj < kv || (kj == kv && sid[j] < v)) ? 1u : 0u;
+        }
+        sorted[s + r] = v;
+      }
+    } else {
+      // two elements per thread and pass, four staged ids per LDS read
+      const uint4* s4 = reinterpret_cast<const uint4*>(sid);
+      for (uint32_t i = threadIdx.x; i < m; i += 2 * blockDim.x) {
+        const uint32_t i1 = i + blockDim.x;
+        const uint32_t v0 = sid[i], v1 = i1 < m ? sid[i1] : 0u;
+        uint32_t r0 = 0, r1 = 0;
+        for (uint32_t j = 0; j < m4; ++j) {
+          const uint4 w = s4[j];
+          r0 += (w.x < v0 ? 1u : 0u) + (w.y < v0 ? 1u : 0u) + (w.z < v0 ? 1u : 0u) + (w.w < v0 ? 1u : 0u);
+          r1 += (w.x < v1 ? 1u : 0u) + (w.y < v1 ? 1u : 0u) + (w.z < v1 ? 1u : 0u) + (w.w < v1 ? 1u : 0u);
+        }
+        sorted[s + r0] = v0;
+        if (i1 < m) sorted[s + r1] = v1;
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+    const uint32_t v = sorted[i];
+    uint32_t r = 0;
+    for (uint32_t j = s; j < e; ++j) r += before(key, sorted[j], v);
+    scratch[s + r] = v;
+  }
+  __syncthreads();
+  for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) sorted[i] = scratch[i];
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ delim,
                                                   const uint32_t* __restrict__ big,
                                                   const uint32_t* __restrict__ nbig,
@@ -304,54 +356,7 @@ __global__ __launch_bounds__(256) void k_sort_big(const uint32_t* __restrict__ d
   const uint32_t nb = *nbig;
   for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
     const uint32_t b = big[q];
-    const uint32_t s = delim[b], e = delim[b + 1], m = e - s;
-    if (m <= kSortLds) {
-      const uint32_t m4 = (m + 3) >> 2;
-      for (uint32_t i = threadIdx.x; i < 4 * m4; i += blockDim.x) {
-        // past the cell: the largest id, which no id of the cell exceeds
-        const uint32_t v = i < m ? sorted[s + i] : 0xffffffffu;
-        sid[i] = v;
-        if (key && i < m) skey[i] = key[v];
-      }
-      __syncthreads();
-      if (key) {
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-          const uint32_t v = sid[i], kv = skey[i];
-          uint32_t r = 0;
-          for (uint32_t j = 0; j < m; ++j) {
-            const uint32_t kj = skey[j];
-            r += (kj < kv || (kj == kv && sid[j] < v)) ? 1u : 0u;
-          }
-          sorted[s + r] = v;
-        }
-      } else {
-        // two elements per thread and pass, four staged ids per LDS read
-        const uint4* s4 = reinterpret_cast<const uint4*>(sid);
-        for (uint32_t i = threadIdx.x; i < m; i += 2 * blockDim.x) {
-          const uint32_t i1 = i + blockDim.x;
-          const uint32_t v0 = sid[i], v1 = i1 < m ? sid[i1] : 0u;
-          uint32_t r0 = 0, r1 = 0;
-          for (uint32_t j = 0; j < m4; ++j) {
-            const uint4 w = s4[j];
-            r0 += (w.x < v0 ? 1u : 0u) + (w.y < v0 ? 1u : 0u) + (w.z < v0 ? 1u : 0u) + (w.w < v0 ? 1u : 0u);
-            r1 += (w.x < v1 ? 1u : 0u) + (w.y < v1 ? 1u : 0u) + (w.z < v1 ? 1u : 0u) + (w.w < v1 ? 1u : 0u);
-          }
-          sorted[s + r0] = v0;
-          if (i1 < m) sorted[s + r1] = v1;
-        }
-      }
-      __syncthreads();
-      continue;
-    }
-    for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) {
-      const uint32_t v = sorted[i];
-      uint32_t r = 0;
-      for (uint32_t j = s; j < e; ++j) r += before(key, sorted[j], v);
-      scratch[s + r] = v;
-    }
-    __syncthreads();
-    for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x) sorted[i] = scratch[i];
-    __syncthreads();
+    rank_sort_cell(sorted, scratch, delim[b], delim[b + 1], key, sid, skey);
   }
 }
 
@@ -364,6 +369,202 @@ __global__ void k_gather(const uint32_t* __restrict__ sorted, int n, const doubl
   sx[k] = x[i];
   sy[k] = y[i];
   sz[k] = z[i];
+}
+
+// ---- many sets at once (a batch): the same passes over every set's
+// points and grid cells in one launch each, a thread finding its set by a
+// binary search over the sets' first entries (base[nsets] = total).
+template <class T>
+__device__ __forceinline__ int set_of(const T* __restrict__ base, int nsets, uint64_t g) {
+  int lo = 0, hi = nsets - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((uint64_t)base[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Bounding boxes of every set: block q takes part q - pblk[set] of its set
+// (grid-stride over the set's points, as k_bounds_partial does for one set)
+// -> part[q][6]; then one wavefront per set reduces its parts and writes its
+// grid (as k_bounds_final).
+__global__ __launch_bounds__(256) void k_seg_bounds_partial(SegBinning a) {
+  __shared__ double sm[6][256];
+  const int t = threadIdx.x;
+  const uint32_t q = blockIdx.x;
+  const int set = set_of(a.pblk, a.nsets, q);
+  const uint32_t r = q - a.pblk[set], nparts = a.pblk[set + 1] - a.pblk[set];
+  const uint64_t p0 = a.pbase[set];
+  const uint32_t n = (uint32_t)(a.pbase[set + 1] - p0);
+  const double* x = a.x + p0;
+  const double* y = a.y + p0;
+  const double* z = a.z + p0;
+  double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+  for (uint32_t i = r * 256 + t; i < n; i += nparts * 256) {
+    const double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = v[k] < lo[k] ? v[k] : lo[k];
+      hi[k] = v[k] > hi[k] ? v[k] : hi[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    sm[k][t] = lo[k];
+    sm[3 + k][t] = hi[k];
+  }
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sm[k][t] = sm[k][t + h] < sm[k][t] ? sm[k][t + h] : sm[k][t];
+        sm[3 + k][t] = sm[3 + k][t + h] > sm[3 + k][t] ? sm[3 + k][t + h] : sm[3 + k][t];
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 6) a.part[6 * (size_t)q + t] = sm[t][0];
+}
+
+__global__ __launch_bounds__(64) void k_seg_bounds_final(SegBinning a) {
+  const int t = threadIdx.x, set = blockIdx.x;
+  const double* part = a.part + 6 * (size_t)a.pblk[set];
+  const int nparts = (int)(a.pblk[set + 1] - a.pblk[set]);
+  double lo[3] = {part[0], part[1], part[2]}, hi[3] = {part[3], part[4], part[5]};
+  for (int b = t; b < nparts; b += 64)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double l = part[6 * b + k], h = part[6 * b + 3 + k];
+      lo[k] = l < lo[k] ? l : lo[k];
+      hi[k] = h > hi[k] ? h : hi[k];
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double l = __shfl_xor(lo[k], o, 64), h = __shfl_xor(hi[k], o, 64);
+      lo[k] = l < lo[k] ? l : lo[k];
+      hi[k] = h > hi[k] ? h : hi[k];
+    }
+  if (t != 0) return;
+  GridDev* g = a.grid + set;
+  const double c = (double)a.cpd[set];
+  g->xmin = lo[0];
+  g->xmax = hi[0];
+  g->ymin = lo[1];
+  g->ymax = hi[1];
+  g->zmin = lo[2];
+  g->zmax = hi[2];
+  g->sx = (hi[0] - lo[0]) / c;
+  g->sy = (hi[1] - lo[1]) / c;
+  g->sz = (hi[2] - lo[2]) / c;
+  g->ix = c / (hi[0] - lo[0]);
+  g->iy = c / (hi[1] - lo[1]);
+  g->iz = c / (hi[2] - lo[2]);
+  g->cpd = a.cpd[set];
+  g->pad = 0;
+}
+
+// Cell ids and the histogram of every set (k_bin_count over the batch; a run
+// of equal ids never crosses sets, whose cells are numbered apart).
+__global__ __launch_bounds__(256) void k_seg_bin_count(SegBinning a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = __lane_id();
+  uint32_t b = 0xffffffffu;  // cell id over the batch
+  if (i < a.n) {
+    const int set = set_of(a.pbase, a.nsets, i);
+    const GridDev g = a.grid[set];
+    const int u = axis_index(a.x[i], g.xmin, g.xmax, g.ix, g.cpd);
+    const int v = axis_index(a.y[i], g.ymin, g.ymax, g.iy, g.cpd);
+    const int w = axis_index(a.z[i], g.zmin, g.zmax, g.iz, g.cpd);
+    const uint32_t local = (uint32_t)((u * g.cpd + v) * g.cpd + w);
+    a.bins[i] = local;
+    b = a.cbase[set] + local;
+  }
+  const uint32_t prev = __shfl_up(b, 1, 64);
+  const bool head = (i < a.n) && (lane == 0 || prev != b);
+  const uint64_t heads = __ballot(head);
+  const uint64_t act = __ballot(i < a.n);
+  uint32_t first = 0;
+  if (head) {
+    const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int last = 63 - __clzll((long long)act);
+    const int nexth = after ? lane + 1 + (__ffsll((unsigned long long)after) - 1) : last + 1;
+    first = atomicAdd(&a.cnt[b], (uint32_t)(nexth - lane));
+  }
+  const uint64_t upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1ull));
+  const int h = upto ? 63 - __clzll((long long)upto) : lane;
+  const uint32_t f = (uint32_t)__shfl((int)first, h, 64);
+  if (i < a.n) a.rank[i] = f + (uint32_t)(lane - h);
+}
+
+// Each set's delimiters from the scan over the whole batch's cells: the
+// scan at a set's cell counts every point of the sets before it too.
+__global__ void k_seg_delim(SegBinning a, uint32_t ndelim) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ndelim) return;
+  const int set = set_of(a.bbase, a.nsets, j);
+  const uint32_t b = j - a.bbase[set], nbs = a.cbase[set + 1] - a.cbase[set];
+  const uint32_t p0 = (uint32_t)a.pbase[set];
+  a.delim[j] = b < nbs ? a.gscan[a.cbase[set] + b] - p0 : (uint32_t)a.pbase[set + 1] - p0;
+}
+
+__global__ void k_seg_scatter(SegBinning a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int set = set_of(a.pbase, a.nsets, i);
+  const uint32_t p0 = (uint32_t)a.pbase[set];
+  a.sorted[p0 + a.delim[a.bbase[set] + a.bins[i]] + a.rank[i]] = i - p0;
+}
+
+// In-cell order of every set's cells (k_sort_cells over the batch); larger
+// cells are listed by batch cell id for k_seg_sort_big.
+__global__ void k_seg_sort_cells(SegBinning a) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.nb) return;
+  const int set = set_of(a.cbase, a.nsets, j);
+  const uint32_t* d = a.delim + a.bbase[set] + (j - a.cbase[set]);
+  const uint32_t s = d[0], e = d[1];
+  if (e - s <= 1) return;
+  if (e - s > 32) {
+    a.big[atomicAdd(&a.misc[1], 1u)] = j;
+    return;
+  }
+  uint32_t* srt = a.sorted + a.pbase[set];
+  for (uint32_t i = s + 1; i < e; ++i) {
+    const uint32_t v = srt[i];
+    uint32_t k = i;
+    while (k > s && v < srt[k - 1]) {
+      srt[k] = srt[k - 1];
+      --k;
+    }
+    srt[k] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seg_sort_big(SegBinning a) {
+  __shared__ __align__(16) uint32_t sid[kSortLds];
+  __shared__ uint32_t skey[kSortLds];
+  const uint32_t nbig = a.misc[1];
+  for (uint32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
+    const uint32_t j = a.big[q];
+    const int set = set_of(a.cbase, a.nsets, j);
+    const uint32_t* d = a.delim + a.bbase[set] + (j - a.cbase[set]);
+    rank_sort_cell(a.sorted + a.pbase[set], a.rank + a.pbase[set], d[0], d[1], nullptr, sid, skey);
+  }
+}
+
+__global__ void k_seg_gather(SegBinning a) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const uint64_t o = a.pbase[set_of(a.pbase, a.nsets, k)] + a.sorted[k];
+  a.sx[k] = a.x[o];
+  a.sy[k] = a.y[o];
+  a.sz[k] = a.z[o];
 }
 
 // Exclusive scan of face counts into u64 offsets: block scan in u64.
@@ -495,6 +696,25 @@ hipError_t launch_counting_sort(const double* x, const double* y, const double* 
                      b.big, b.misc + 1, key);
   hipLaunchKernelGGL(k_sort_big, dim3(1024), dim3(256), 0, s, b.delim, b.big, b.misc + 1, b.sorted,
                      b.scratch, key);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_binning(const SegBinning& a, hipStream_t s) {
+  if (a.nsets <= 0 || a.n == 0) return hipSuccess;
+  // fill kernels rather than memset nodes: this sequence may be captured
+  hipError_t e;
+  if ((e = launch_fill_u32(a.cnt, a.nb, 0u, s)) != hipSuccess || (e = launch_fill_u32(a.misc, 2, 0u, s)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_seg_bounds_partial, dim3(a.nparts), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_seg_bounds_final, dim3(a.nsets), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_seg_bin_count, dim3(blocks(a.n, 256)), dim3(256), 0, s, a);
+  if ((e = launch_scan_u32(a.cnt, (int)a.nb, a.gscan, a.sums, a.misc, s)) != hipSuccess) return e;
+  const uint32_t ndelim = a.nb + (uint32_t)a.nsets;
+  hipLaunchKernelGGL(k_seg_delim, dim3(blocks(ndelim, 256)), dim3(256), 0, s, a, ndelim);
+  hipLaunchKernelGGL(k_seg_scatter, dim3(blocks(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_seg_sort_cells, dim3(blocks(a.nb, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_seg_sort_big, dim3(1024), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_seg_gather, dim3(blocks(a.n, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -24,6 +24,36 @@ struct SortBufs {
 hipError_t launch_counting_sort(const double* x, const double* y, const double* z, int n,
                                 const GridDev* g, int nbins, const SortBufs& b, hipStream_t s,
                                 const uint32_t* key = nullptr);
+// The counting sort of many point sets at once (a batch): set after set in
+// the SoA arrays, their grid cells numbered apart (set k's cells from
+// cbase[k]), their delimiters and sorted indices set-local, as
+// launch_bounds + launch_counting_sort + launch_gather would leave them set
+// by set (input order inside a cell).  pbase, cbase, bbase: nsets + 1
+// entries (first point / cell / delimiter of each set, then the totals).
+struct SegBinning {
+  int nsets = 0;
+  uint32_t n = 0, nb = 0;  // points, grid cells over all sets
+  const uint64_t* pbase = nullptr;
+  const uint32_t* cbase = nullptr;
+  const uint32_t* bbase = nullptr;
+  const int32_t* cpd = nullptr;  // cells per dimension of each set
+  const uint32_t* pblk = nullptr;  // [nsets + 1] first bounds block of each set
+  uint32_t nparts = 0;             // bounds blocks over all sets
+  double* part = nullptr;          // [nparts][6] their partial boxes
+  const double *x = nullptr, *y = nullptr, *z = nullptr;
+  GridDev* grid = nullptr;  // [nsets]
+  uint32_t* bins = nullptr;     // [n] set-local cell of each point
+  uint32_t* rank = nullptr;     // [n] slot in its cell, then the large cells' scratch
+  uint32_t* cnt = nullptr;      // [nb]
+  uint32_t* gscan = nullptr;    // [nb] exclusive scan of cnt over the batch
+  uint32_t* sums = nullptr;     // [ceil(nb / 1024) + 1]
+  uint32_t* delim = nullptr;    // [nb + nsets]
+  uint32_t* sorted = nullptr;   // [n]
+  uint32_t* big = nullptr;      // [nb]
+  uint32_t* misc = nullptr;     // [2]: scan total, large cells
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr;  // [n] positions in grid order
+};
+hipError_t launch_seg_binning(const SegBinning& a, hipStream_t s);
 // The same binning without the in-cell ordering: b.sorted lists the n
 // positions grid cell by grid cell, in no set order inside a cell (only
 // bins, cnt, delim, cursor, sorted, sums and misc are used)
