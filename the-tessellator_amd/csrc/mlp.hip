@@ -69,9 +69,16 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* __restrict__ 
 
 // Features: log(1 + count) of the 3x3x3 grid cells around each point's grid
 // cell (offsets -1..1, x outermost), 0 outside the grid; bf16, 32 per point
-// (the last 5 are 0).  bins: grid cell of each point (original order).
-__global__ void k_features(const uint32_t* __restrict__ bins, const uint32_t* __restrict__ delim, int cpd, int n,
-                           uint16_t* __restrict__ feat) {
+// (the last 5 are 0).  bins: grid cell of each point (original order).  The
+// values of counts below kFeatTab are computed once per block into LDS by
+// the same expression (so bit for bit the same) and looked up.
+constexpr int kFeatTab = 1024;
+__global__ __launch_bounds__(256) void k_features(const uint32_t* __restrict__ bins,
+                                                  const uint32_t* __restrict__ delim, int cpd, int n,
+                                                  uint16_t* __restrict__ feat) {
+  __shared__ uint16_t tab[kFeatTab];
+  for (int c = threadIdx.x; c < kFeatTab; c += blockDim.x) tab[c] = to_bf16((float)log1p((double)c));
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b0 = (int)bins[i];
@@ -86,7 +93,7 @@ __global__ void k_features(const uint32_t* __restrict__ bins, const uint32_t* __
         const int b = (bx * cpd + by) * cpd + bz;
         c = delim[b + 1] - delim[b];
       }
-      f[j] = to_bf16((float)log1p((double)c));
+      f[j] = c < (uint32_t)kFeatTab ? tab[c] : to_bf16((float)log1p((double)c));
     } else {
       f[j] = 0;
     }
