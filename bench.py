@@ -283,6 +283,8 @@ def bench_clustered(ctx, w, args):
         "config_sweep": sweep,
         "model": train,
     }
+    # host memory to host memory on the path just timed (untimed for the value)
+    extra["end_to_end"] = end_to_end(ctx, pts, lo, hi, w["density"], batch_config=cfg, model=m)
     mode = "learned-batched" if cfg["learned"] else "batched (the tuner kept the plain list order)"
     if not cfg["learned"]:
         extra["fallback_cells"] = extra["fallback_rate"] = None
@@ -340,21 +342,30 @@ def bench_batch(ctx, w, args):
     return sets[0][0], lo, hi, n, ms, tims, None, extra, {"mode": "batched", "sets": len(sets)}
 
 
-def end_to_end(ctx, pts, lo, hi, density, reps=3):
-    """The plain path from host memory to host memory: upload (create), the
-    compute, and the download of every cell's results (PCIe both ways)."""
+def end_to_end(ctx, pts, lo, hi, density, reps=3, batch_config=None, model=None):
+    """The timed path from host memory to host memory: upload (create), the
+    compute, and the download of every cell's results (PCIe both ways); the
+    batched path with the given configuration when one is given, else the
+    plain one."""
     best = None
     for _ in range(reps):
         ctx.synchronize()
         t0 = time.perf_counter()
-        d = tess.Diagram(ctx, pts, lo, hi, density=density)
-        d.compute()
-        d.cells()
+        if batch_config is not None:
+            d = tess.Batch(ctx, [(pts, lo, hi)], density=density)
+            d.set_config(**batch_config)
+            d.compute(model)
+            d.cells(0)
+        else:
+            d = tess.Diagram(ctx, pts, lo, hi, density=density)
+            d.compute()
+            d.cells()
         t1 = time.perf_counter()
         d.close()
         best = t1 - t0 if best is None else min(best, t1 - t0)
+    path = "batched path with the timed configuration" if batch_config is not None else "plain path"
     return {"ms": best * 1e3, "cells_per_s": len(pts) / best,
-            "includes": "host points -> HBM, compute, every cell's results HBM -> host (best of 3)"}
+            "includes": f"host points -> HBM, compute ({path}), every cell's results HBM -> host (best of 3)"}
 
 
 def bench_fcc(ctx, w, args):
@@ -765,7 +776,8 @@ def main():
         line["roofline"]["alg_hbm_gbs"] = (44 * n + 12 * faces) / (kernel_ms / 1e3) / 1e9
     line.update(extra)
     if w["kind"] in ("clustered", "uniform") and not args.plain_only:
-        line["end_to_end"] = end_to_end(ctx, pts, lo, hi, w["density"])
+        if "end_to_end" not in line:
+            line["end_to_end"] = end_to_end(ctx, pts, lo, hi, w["density"])
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(pts, lo, hi, w["density"], args.cpu_frac, threads)
