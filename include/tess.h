@@ -177,8 +177,8 @@ tess_status tess_diagram_compute(tess_diagram* d);
 tess_status tess_diagram_total_faces(const tess_diagram* d, uint64_t* out);
 /* Copy results to host arrays: volume[n], face_count[n], face_offset[n+1],
  * neighbor[total_faces], face_area[total_faces]; any pointer may be NULL.
- * Arrays of 128 MB or more (here and in every other *_copy* call) come down
- * through two 64 MB pinned host buffers the context allocates on first use
+ * Arrays of 16 MiB or more (here and in every other *_copy* call) come down
+ * through two 64 MiB pinned host buffers the context allocates on first use
  * and keeps until tess_ctx_destroy, copied out by up to eight host threads;
  * calls on one context take the buffers in turn. */
 tess_status tess_diagram_copy_cells(const tess_diagram* d, double* volume, uint32_t* face_count,

@@ -142,12 +142,15 @@ void host_copy(char* dst, const char* src, size_t len) {
 }
 
 hipError_t download(tess_ctx* c, void* dst, const void* src, size_t bytes) {
-  constexpr size_t kChunk = Staging::kBytes;
-  if (bytes < 2 * kChunk) return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  constexpr size_t kMin = size_t(16) << 20;
+  if (bytes < kMin) return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  // at least four chunks (so that transfers and host copies overlap), of
+  // 8 MiB to one staging buffer each
+  const size_t kChunk = std::min(Staging::kBytes, std::max(size_t(8) << 20, (bytes / 4 + 4095) & ~size_t(4095)));
   Staging& S = c->stage;
   std::lock_guard<std::mutex> g(S.mu);
   for (int k = 0; k < 2; ++k) {
-    if (!S.buf[k] && hipHostMalloc(&S.buf[k], kChunk, hipHostMallocDefault) != hipSuccess) {
+    if (!S.buf[k] && hipHostMalloc(&S.buf[k], Staging::kBytes, hipHostMallocDefault) != hipSuccess) {
       S.buf[k] = nullptr;
       return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);  // no pinned memory: the plain copy
     }

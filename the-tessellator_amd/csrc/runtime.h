@@ -285,9 +285,10 @@ struct tess_model {
 namespace tess {
 
 // dst <- src (device), bytes of it, ordered after the context stream's work
-// and finished on return.  Large copies go through the context's pinned
-// staging, chunk by chunk, overlapping the transfer of one chunk with the
-// host copy of the one before; small ones are one hipMemcpy.
+// and finished on return.  Copies of 16 MiB or more go through the
+// context's pinned staging, chunk by chunk, overlapping the transfer of one
+// chunk with the host copy of the one before; smaller ones are one
+// hipMemcpy.
 hipError_t download(tess_ctx* c, void* dst, const void* src, size_t bytes);
 // dst <- src, len bytes, on up to eight host threads in pieces of at least
 // 4 MiB (the staged download's copy out of pinned memory)
